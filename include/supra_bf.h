@@ -1,0 +1,198 @@
+/*
+ * supra_bf.h — C ABI of the MI355X (gfx950) receive beamformer.
+ *
+ * This library replaces the CUDA engines behind SUPRA's BeamformingNode,
+ * RawDelayNode and BeamformingMVNode.  Every entry point below names the
+ * reference interface it stands in for (paths relative to the SUPRA source
+ * tree).  The ABI uses plain pointers, sizes and POD structs only: device
+ * buffers are passed as `void*` / typed pointers that the caller owns, and
+ * HIP streams are passed as `void*` (a `hipStream_t`; NULL = default stream).
+ *
+ * Conventions shared by all entry points
+ *   - Return value: SUPRA_OK (0) on success, a negative SUPRA_ERR_* code
+ *     otherwise; supra_last_error() returns a message for the calling thread.
+ *   - Launches are asynchronous on `stream`; nothing synchronises unless the
+ *     function says so (the reference likewise returns without a sync,
+ *     src/SupraLib/Beamformer/RxBeamformerCuda.cu:551-558).
+ *   - RF layout [tx][channel][sample], sample fastest
+ *     (RxSampleBeamformerDelayAndSum.h:181).  Image layout [depth][scanline],
+ *     scanline fastest (RxBeamformerCuda.cu:321).  RawDelayed layout
+ *     [scanline][channel][depth], depth fastest (RawDelay.cu:307).
+ *   - Lengths in the geometry are millimetres and seconds, exactly as
+ *     RxBeamformerParameters holds them; the library converts to "dt-space"
+ *     (samples of one-way travel) itself on every call whose sampling
+ *     frequency or speed of sound differs from the last one.
+ */
+#ifndef SUPRA_BF_H
+#define SUPRA_BF_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SUPRA_OK                 0
+#define SUPRA_ERR_INVALID_ARG   -1
+#define SUPRA_ERR_UNSUPPORTED   -2
+#define SUPRA_ERR_HIP           -3
+#define SUPRA_ERR_NO_DEVICE     -4
+#define SUPRA_ERR_TOO_LARGE     -5
+
+/* ---- element types (USRawData / USImage payloads) ---------------------- */
+#define SUPRA_INT16    1   /* int16_t: RF from the probe, or clamped image */
+#define SUPRA_FLOAT32  2   /* float */
+
+/* ---- WindowType (src/SupraLib/Beamformer/WindowFunction.h:37-44) ------- */
+#define SUPRA_WINDOW_RECTANGULAR 0
+#define SUPRA_WINDOW_HANN        1
+#define SUPRA_WINDOW_HAMMING     2
+#define SUPRA_WINDOW_GAUSS       3
+
+/* ---- RxSampleBeamformer (src/SupraLib/Beamformer/RxBeamformerCuda.h:31-37) */
+#define SUPRA_BF_DELAY_AND_SUM   0
+#define SUPRA_BF_TEST_SIGNAL     3
+
+/* Number of window-LUT entries (RxBeamformerCuda.h:78). */
+#define SUPRA_WINDOW_NUM_ENTRIES 64
+
+/* Marker of an element that is connected to no channel
+ * (src/SupraLib/Beamformer/USTransducer.h:35). */
+#define SUPRA_ELEMENT_NOT_CONNECTED (-1)
+
+/* ---- ScanlineRxParameters3D (src/SupraLib/USImageProperties.h:43-95) ----
+ * Field for field the reference struct, in an explicit C layout.  Positions
+ * in mm, initial_delay in seconds.  `first`/`last` are the producer's
+ * inclusive aperture corners; the beamformers loop first <= e < last
+ * (RxSampleBeamformerDelayAndSum.h:55-57,150). */
+typedef struct supra_tx_params {
+    uint16_t first_active_element[2];   /* x, y */
+    uint16_t last_active_element[2];    /* x, y */
+    uint16_t tx_scanline_idx;
+    uint16_t reserved[3];
+    double   initial_delay;             /* [s] */
+} supra_tx_params;
+
+typedef struct supra_scanline_rx {
+    double position[3];                 /* x, y(depth axis), z(elevation) [mm] */
+    double direction[3];                /* unit vector, depth component in [1] */
+    double tx_weights[4];
+    supra_tx_params tx[4];
+    double max_element_distance[2];     /* [mm] */
+} supra_scanline_rx;
+
+/* ---- RxBeamformerParameters (src/SupraLib/Beamformer/RxBeamformerParameters.h:38-112)
+ * Host arrays; the library copies them in supra_bf_create. */
+typedef struct supra_rx_geometry {
+    uint32_t num_rx_scanlines;          /* rx_scanline_layout[0] * [1] */
+    uint32_t rx_scanline_layout[2];     /* 3D iff both > 1 (RxBeamformerCuda.cu:31) */
+    uint32_t num_depths;
+    uint32_t num_elements;
+    uint32_t element_layout[2];         /* elements per row / rows */
+    const supra_scanline_rx* scanlines; /* num_rx_scanlines, x fastest */
+    const float* depths;                /* num_depths [mm] */
+    const float* element_x;             /* num_elements [mm] */
+    const float* element_y;             /* num_elements [mm] (the "z" of the kernels) */
+    const int32_t* element_to_channel;  /* NULL: channel = element % num_channels */
+    double speed_of_sound_mm_per_s;     /* the geometry's c */
+    double depth;                       /* imaging depth [mm] */
+} supra_rx_geometry;
+
+/* ---- Beamformer configuration (src/SupraLib/Beamformer/Beamformer.h, setters
+ *      Beamformer.cpp:91-360) and USTransducer (USTransducer.cpp:23-49). ---- */
+#define SUPRA_SCAN_LINEAR     0
+#define SUPRA_SCAN_PHASED     1
+#define SUPRA_SCAN_BIPHASED   2
+#define SUPRA_SCAN_PLANEWAVE  3
+#define SUPRA_TRANSDUCER_LINEAR 0
+#define SUPRA_TRANSDUCER_PLANAR 1
+
+typedef struct supra_beamformer_config {
+    int32_t  scan_type;                 /* SUPRA_SCAN_* */
+    int32_t  transducer_type;           /* SUPRA_TRANSDUCER_* */
+    uint32_t element_layout[2];         /* linear: {N, 1} */
+    double   pitch[2];                  /* uniform element pitch [mm] (x, y) */
+    uint32_t num_scanlines[2];          /* transmit events (x, y) */
+    uint32_t rx_scanline_subdivision[2];
+    uint32_t max_aperture[2];           /* receive aperture (elements) */
+    uint32_t tx_max_aperture[2];        /* transmit aperture (elements) */
+    double   depth;                     /* [mm] */
+    uint32_t num_depths;                /* reconstructed samples per scanline */
+    double   steering_angle_deg[2];
+    double   sector_angle_deg[2];
+    int32_t  tx_focus_active;
+    double   tx_focus_depth;            /* [mm] */
+    double   tx_focus_width;            /* [mm] */
+    double   speed_of_sound;            /* [m/s] */
+} supra_beamformer_config;
+
+/* Size query for supra_geometry_build: number of rx scanlines and elements. */
+int supra_geometry_sizes(const supra_beamformer_config* cfg,
+                         uint32_t* num_rx_scanlines, uint32_t rx_layout[2],
+                         uint32_t* num_elements);
+
+/* Beamformer::computeTxParameters + getCurrentRxBeamformerParameters
+ * (src/SupraLib/Beamformer/Beamformer.cpp:409-838, RxBeamformerParameters.cpp:38-88,
+ *  USTransducer::computeInternals USTransducer.cpp:179-258).
+ * Fills caller arrays: scanlines[num_rx_scanlines], depths[num_depths],
+ * element_x/element_y[num_elements]. */
+int supra_geometry_build(const supra_beamformer_config* cfg,
+                         supra_scanline_rx* scanlines, float* depths,
+                         float* element_x, float* element_y);
+
+/* ---- engine context ------------------------------------------------------ */
+typedef struct supra_bf_ctx supra_bf_ctx;
+
+/* RxBeamformerCuda::RxBeamformerCuda(const RxBeamformerParameters&)
+ * (src/SupraLib/Beamformer/RxBeamformerCuda.cu:24-52); also stands in for
+ * RawDelay::RawDelay (RawDelay.cu:34-60).  Copies the geometry to the
+ * current HIP device. */
+int  supra_bf_create(const supra_rx_geometry* geometry, supra_bf_ctx** out_ctx);
+void supra_bf_destroy(supra_bf_ctx* ctx);
+
+/* RxBeamformerCuda::performRxBeamforming<ChannelDataType, ImageDataType>
+ * (src/SupraLib/Beamformer/RxBeamformerCuda.cu:539-663; kernels :109-323).
+ * rf:  device [num_tx][num_channels][num_samples] of rf_dtype
+ * out: device [num_depths][num_rx_scanlines] of out_dtype (caller-owned)
+ * sample_beamformer: SUPRA_BF_DELAY_AND_SUM or SUPRA_BF_TEST_SIGNAL.
+ * Linear interpolation of the RF is always on, as in the reference
+ * (RxBeamformerCuda.cu:598,623). */
+int supra_das_run(supra_bf_ctx* ctx, int32_t sample_beamformer,
+                  const void* rf, int32_t rf_dtype,
+                  uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                  double sampling_frequency_hz,
+                  double f_number, double speed_of_sound_mm_per_s,
+                  int32_t window_type, float window_parameter,
+                  int32_t interpolate_transmits, int32_t additional_offset,
+                  void* out, int32_t out_dtype, void* stream);
+
+/* The same as supra_das_run for `num_frames` frames that share geometry and
+ * parameters: rf is [frame][tx][ch][s], out [frame][depth][scanline].
+ * No reference counterpart (the reference beamforms one USRawData per call,
+ * BeamformingNode.cpp:128-177); it exists so that short frames fill the GPU. */
+int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer, uint32_t num_frames,
+                         const void* rf, int32_t rf_dtype,
+                         uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                         double sampling_frequency_hz,
+                         double f_number, double speed_of_sound_mm_per_s,
+                         int32_t window_type, float window_parameter,
+                         int32_t interpolate_transmits, int32_t additional_offset,
+                         void* out, int32_t out_dtype, void* stream);
+
+/* Window LUT of the reference (WindowFunction.cpp:18-47, WindowFunction.h:122-138),
+ * computed on the host: SUPRA_WINDOW_NUM_ENTRIES floats, max-normalised. */
+int supra_window_lut(int32_t window_type, float window_parameter, float* lut_out);
+
+/* Last error message of the calling thread ("" if none). */
+const char* supra_last_error(void);
+
+/* Library version string. */
+const char* supra_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUPRA_BF_H */

@@ -1,0 +1,414 @@
+// ============================================================================
+// oracle.cpp — CPU restatement of SUPRA's receive beamformer.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke test
+// of __graft_entry__ and the cpu_baseline leg of bench.py call it to produce
+// the expected output of the HIP library on the same inputs.  Nothing in the
+// product (supra_amd/) links, loads or calls it.
+//
+// Two precisions of the same algorithm:
+//   * FP32: the reference's single-precision arithmetic restated literally —
+//     same operand order, IEEE rounding, no FMA contraction (built with
+//     -ffp-contract=off), fmaxf/fminf clamps that keep the non-NaN operand,
+//     CUDA's saturating float->uint32 conversion where the reference relies on
+//     it.  The HIP kernels are expected to match it bit for bit.
+//   * FP64: the same intended math in double precision end to end (geometry
+//     conversion, window table, delays, interpolation, sums): the accuracy
+//     yardstick the FP32 results are reported against.
+//
+// Each function cites the reference lines it follows (paths relative to the
+// SUPRA source tree).  What pins each function is listed in its header
+// comment; a function without such a list would be "parity unpinned".
+// ============================================================================
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "supra_bf.h"
+
+namespace {
+
+constexpr int kEntries = 64;
+
+// Static partition of [0, n) over worker threads.
+template <typename Fn>
+void parallel_for(int64_t n, int nthreads, Fn fn) {
+    if (nthreads <= 1 || n < 64) {
+        for (int64_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    int64_t per = (n + nthreads - 1) / nthreads;
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nthreads; w++) {
+        int64_t lo = w * per, hi = std::min<int64_t>(n, lo + per);
+        if (lo >= hi) break;
+        pool.emplace_back([=]() {
+            for (int64_t i = lo; i < hi; i++) fn(i);
+        });
+    }
+    for (auto& t : pool) t.join();
+}
+
+// --- scalar helpers with the reference's semantics, per precision ------------
+inline float vmax(float a, float b) { return std::fmax(a, b); }
+inline float vmin(float a, float b) { return std::fmin(a, b); }
+inline double vmax(double a, double b) { return std::fmax(a, b); }
+inline double vmin(double a, double b) { return std::fmin(a, b); }
+inline float vsqrt(float a) { return std::sqrt(a); }
+inline double vsqrt(double a) { return std::sqrt(a); }
+inline float vfloor(float a) { return std::floor(a); }
+inline double vfloor(double a) { return std::floor(a); }
+inline float vround(float a) { return std::round(a); }
+inline double vround(double a) { return std::round(a); }
+inline float vcos(float a) { return std::cos(a); }
+inline double vcos(double a) { return std::cos(a); }
+
+// WindowFunction::windowFunction (src/SupraLib/Beamformer/WindowFunction.h:121-138)
+// + normalisation (WindowFunction.cpp:18-47).
+// Pinned by: tests/test_oracle_window.py — closed-form window values computed
+// in numpy float64, max == 1 after normalisation, end points of Hann/Hamming,
+// Gauss symmetry; golden tables in tests/golden/window_luts.json.
+template <typename T>
+void window_lut(int type, float p_in, T* lut) {
+    const T p = static_cast<T>(p_in);
+    T vmax_ = std::numeric_limits<T>::min();
+    for (int i = 0; i < kEntries; i++) {
+        T r = (static_cast<T>(i) / static_cast<T>(kEntries - 1)) * 2 - 1;
+        T v;
+        const T two_pi = 2 * static_cast<T>(M_PI);
+        switch (type) {
+        case SUPRA_WINDOW_RECTANGULAR: v = 1; break;
+        case SUPRA_WINDOW_HANN: v = (1 - p) * (T(0.5) - T(0.5) * vcos(two_pi * ((r + 1) * T(0.5)))) + p; break;
+        case SUPRA_WINDOW_HAMMING: v = (1 - p) * (T(0.54) - T(0.46) * vcos(two_pi * ((r + 1) * T(0.5)))) + p; break;
+        case SUPRA_WINDOW_GAUSS: {
+            T q = r / p;
+            v = static_cast<T>(1.0 / (p * std::sqrt(2.0 * M_PI)) * std::exp((-1.0 / 2.0) * q * q));
+            break;
+        }
+        default: v = 0;
+        }
+        lut[i] = v;
+        vmax_ = std::max(vmax_, v);
+    }
+    for (int i = 0; i < kEntries; i++) lut[i] /= vmax_;
+}
+
+// WindowFunctionGpu::get (WindowFunction.h:58-64)
+template <typename T>
+inline T window_get(const T* lut, T rel) {
+    T cl = vmin(vmax(rel, T(-1)), T(1));
+    T idx = vround(T(31.5) * (cl + T(1)));
+    return lut[static_cast<uint32_t>(idx)];
+}
+
+template <typename T>
+inline T clamp_out(T x, int out_dtype) {
+    if (out_dtype == SUPRA_INT16) {
+        // clampCast<int16_t> (utilities/cudaUtility.h:111-147): truncation.
+        return static_cast<T>(static_cast<int16_t>(vmin(vmax(x, T(-32767)), T(32767))));
+    }
+    return vmin(vmax(x, T(-FLT_MAX)), T(FLT_MAX));
+}
+
+// Transmit selection (RxBeamformerCuda.cu:274-295): running max held in float
+// and compared against the double weights, ties keep the lowest index.
+int select_tx(const supra_scanline_rx& s) {
+    int best = 0;
+    float best_w = static_cast<float>(s.tx_weights[0]);
+    for (int k = 1; k < 4; k++) {
+        if (s.tx_weights[k] > best_w) {
+            best_w = static_cast<float>(s.tx_weights[k]);
+            best = k;
+        }
+    }
+    return best;
+}
+
+// dt-space geometry of RxBeamformerCuda::convertToDtSpace (RxBeamformerCuda.cu:58-107),
+// as the kernels read it: float tables for FP32, doubles for FP64.
+template <typename T>
+struct DtGeometry {
+    std::vector<T> depth, ex, ey;
+    std::vector<T> sx, sz, dirx, diry, dirz, medx, medz;
+    std::vector<T> t0;   // [scanline*4 + k]
+};
+
+template <typename T>
+DtGeometry<T> to_dt_space(const supra_rx_geometry& g, double dt, double c) {
+    DtGeometry<T> o;
+    const double factor = 1 / 1.0 / (c * dt);
+    const double factor_time = 1 / 1.0 / dt;
+    o.depth.resize(g.num_depths);
+    for (uint32_t i = 0; i < g.num_depths; i++) o.depth[i] = static_cast<T>(g.depths[i] * factor);
+    o.ex.resize(g.num_elements);
+    o.ey.resize(g.num_elements);
+    for (uint32_t i = 0; i < g.num_elements; i++) {
+        o.ex[i] = static_cast<T>(g.element_x[i] * factor);
+        o.ey[i] = static_cast<T>(g.element_y[i] * factor);
+    }
+    const uint32_t n = g.num_rx_scanlines;
+    o.sx.resize(n); o.sz.resize(n); o.dirx.resize(n); o.diry.resize(n); o.dirz.resize(n);
+    o.medx.resize(n); o.medz.resize(n); o.t0.resize(4 * n);
+    for (uint32_t i = 0; i < n; i++) {
+        const supra_scanline_rx& s = g.scanlines[i];
+        o.sx[i] = static_cast<T>(s.position[0] * factor);
+        o.sz[i] = static_cast<T>(s.position[2] * factor);
+        o.dirx[i] = static_cast<T>(s.direction[0]);
+        o.diry[i] = static_cast<T>(s.direction[1]);
+        o.dirz[i] = static_cast<T>(s.direction[2]);
+        o.medx[i] = static_cast<T>(s.max_element_distance[0] * factor);
+        o.medz[i] = static_cast<T>(s.max_element_distance[1] * factor);
+        for (int k = 0; k < 4; k++) o.t0[4 * i + k] = static_cast<T>(s.tx[k].initial_delay * factor_time);
+    }
+    return o;
+}
+
+struct RfView {
+    const void* data;
+    int dtype;
+    uint32_t n_tx, n_ch, n_s;
+    inline double at(uint64_t idx) const {
+        return dtype == SUPRA_INT16 ? static_cast<const int16_t*>(data)[idx] : static_cast<const float*>(data)[idx];
+    }
+};
+
+template <typename T>
+inline T rf_at(const RfView& rf, uint64_t idx) {
+    if (rf.dtype == SUPRA_INT16) return static_cast<T>(static_cast<const int16_t*>(rf.data)[idx]);
+    return static_cast<T>(static_cast<const float*>(rf.data)[idx]);
+}
+
+struct DasCall {
+    const supra_rx_geometry* g;
+    RfView rf;
+    double fs, F, c;
+    int window;
+    float wparam;
+    int interp_tx;
+    int32_t offset;
+    int testsig;
+    int out_dtype;
+};
+
+// RxSampleBeamformerTestSignal (RxSampleBeamformerTestSignal.h:25-133).
+template <typename T>
+T test_signal(bool is3d, T x, T z, T dirx, T diry, T dirz, T d, T dt, T c) {
+    T px = ((x + dirx * d) * dt) * c;
+    T py = ((diry * d) * dt) * c;
+    T pz = is3d ? ((z + dirz * d) * dt) * c : ((dirz * d) * dt) * c;
+    T sample = 0;
+    for (int k = -3; k <= 3; k++) {
+        T ex = px - static_cast<T>(k) * T(6), ey = py - T(30), ez = T(0);
+        if (vsqrt(ex * ex + ey * ey + ez * ez) <= T(2)) sample = T(1000);
+    }
+    if (is3d) {
+        for (int k = -3; k <= 3; k++) {
+            T ex = T(0), ey = py - T(30), ez = pz - static_cast<T>(k) * T(6);
+            if (vsqrt(ex * ex + ey * ey + ez * ez) <= T(2)) sample = T(1000);
+        }
+    }
+    (void)pz;
+    return sample;
+}
+
+// One pixel of 2D DAS: rxBeamformingDTSPACEKernel (RxBeamformerCuda.cu:230-323)
+// with RxSampleBeamformerDelayAndSum::sampleBeamform2D
+// (RxSampleBeamformerDelayAndSum.h:122-208) and computeDelayDTSPACE_D
+// (RxBeamformerCommon.h:27-33).
+// Pinned by: tests/test_oracle_das.py — constant RF (out = k*n), linear ramp
+// RF (out = n * weighted mean of the closed-form delays), linearity, a point
+// scatterer synthesised in millimetres focusing at its own pixel (pins the
+// dt-space scaling and the two-way delay), the TestSignal known answer, FP32
+// vs FP64 agreement, brute force on tiny inputs against a pure-Python loop.
+template <typename T>
+T das_pixel_2d(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32_t s, uint32_t r) {
+    const supra_rx_geometry& g = *call.g;
+    const supra_scanline_rx& sc = g.scanlines[s];
+    const T dt = static_cast<T>(1.0 / call.fs), c = static_cast<T>(call.c), F = static_cast<T>(call.F);
+    const T d = G.depth[r];
+    const T aDT = (((d * dt) * c) / (T(2) * F)) / c / dt;
+    const T x = G.sx[s], dirx = G.dirx[s], diry = G.diry[s], dirz = G.dirz[s];
+    const T inv = T(1) / vmin(aDT, G.medx[s]);
+    const T off = static_cast<T>(call.offset);
+    const int sel = select_tx(sc);
+    T s_interp = 0;
+    for (int k = 0; k < 4; k++) {
+        bool use = call.interp_tx ? (sc.tx_weights[k] > 0.0) : (k == sel && sc.tx_weights[k] > 0.0);
+        if (!use) continue;
+        const supra_tx_params& tp = sc.tx[k];
+        T s_local;
+        if (call.testsig) {
+            s_local = test_signal<T>(false, x, T(0), dirx, diry, dirz, d, dt, c);
+        } else {
+            const T t0 = G.t0[4 * s + k];
+            T sample = 0, wsum = 0;
+            int nadd = 0;
+            const uint64_t tx_base = static_cast<uint64_t>(tp.tx_scanline_idx) * call.rf.n_ch * call.rf.n_s;
+            for (int32_t e = tp.first_active_element[0]; e < tp.last_active_element[0]; e++) {
+                uint32_t ch;
+                if (g.element_to_channel) {
+                    if (g.element_to_channel[e] < 0) continue;
+                    ch = static_cast<uint32_t>(g.element_to_channel[e]);
+                } else {
+                    ch = static_cast<uint32_t>(e) % call.rf.n_ch;
+                }
+                const T xe = G.ex[e];
+                if (std::fabs(xe - x) <= aDT) {
+                    const T w = window_get(lut, (xe - x) * inv);
+                    wsum += w;
+                    nadd++;
+                    const T a = xe - (x + dirx * d);
+                    const T b = diry * d;
+                    T tau = t0 + (vsqrt(a * a + b * b) + d);
+                    tau = tau + off;
+                    const int32_t i = static_cast<int32_t>(vfloor(tau));
+                    const T f = tau - static_cast<T>(i);
+                    const uint64_t base = tx_base + static_cast<uint64_t>(ch) * call.rf.n_s;
+                    if (static_cast<uint32_t>(i) < call.rf.n_s - 1) {
+                        const T v0 = rf_at<T>(call.rf, base + static_cast<uint32_t>(i));
+                        const T v1 = rf_at<T>(call.rf, base + static_cast<uint32_t>(i) + 1);
+                        sample += w * ((T(1) - f) * v0 + f * v1);
+                    } else if (static_cast<uint32_t>(i) < call.rf.n_s && f == T(0)) {
+                        sample += w * rf_at<T>(call.rf, base + static_cast<uint32_t>(i));
+                    }
+                }
+            }
+            s_local = (nadd > 0) ? (sample / wsum) * static_cast<T>(nadd) : T(0);
+        }
+        if (call.interp_tx) s_interp += static_cast<T>(static_cast<float>(sc.tx_weights[k])) * s_local;
+        else s_interp += s_local;
+    }
+    return clamp_out(s_interp, call.out_dtype);
+}
+
+// One pixel of 3D DAS: rxBeamformingDTSPACE3DKernel (RxBeamformerCuda.cu:109-228),
+// sampleBeamform3D (RxSampleBeamformerDelayAndSum.h:25-120),
+// computeDelayDTSPACE3D_D / computeWindow3D (RxBeamformerCommon.h:35-61).
+// Pinned by: the same property tests as 2D run on a matrix-array geometry
+// (tests/test_oracle_das.py::test_3d_*), plus the square aperture radius.
+template <typename T>
+T das_pixel_3d(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32_t s, uint32_t r) {
+    const supra_rx_geometry& g = *call.g;
+    const supra_scanline_rx& sc = g.scanlines[s];
+    const T dt = static_cast<T>(1.0 / call.fs), c = static_cast<T>(call.c), F = static_cast<T>(call.F);
+    const T d = G.depth[r];
+    const T t_ap = (((d * dt) * c) / (T(2) * F)) / c / dt;
+    const T aDT = t_ap * t_ap;
+    const T x = G.sx[s], z = G.sz[s], dirx = G.dirx[s], diry = G.diry[s], dirz = G.dirz[s];
+    const T sq = vsqrt(aDT);
+    const T invx = T(1) / vmin(sq, G.medx[s]);
+    const T invz = T(1) / vmin(sq, G.medz[s]);
+    const T off = static_cast<T>(call.offset);
+    const int sel = select_tx(sc);
+    const uint32_t lx = g.element_layout[0];
+    T s_interp = 0;
+    for (int k = 0; k < 4; k++) {
+        bool use = call.interp_tx ? (sc.tx_weights[k] > 0.0) : (k == sel && sc.tx_weights[k] > 0.0);
+        if (!use) continue;
+        const supra_tx_params& tp = sc.tx[k];
+        T s_local;
+        if (call.testsig) {
+            s_local = test_signal<T>(true, x, z, dirx, diry, dirz, d, dt, c);
+        } else {
+            const T t0 = G.t0[4 * s + k];
+            T sample = 0, wsum = 0;
+            int nadd = 0;
+            const uint64_t tx_base = static_cast<uint64_t>(tp.tx_scanline_idx) * call.rf.n_ch * call.rf.n_s;
+            for (uint32_t ex = tp.first_active_element[0]; ex < tp.last_active_element[0]; ex++) {
+                for (uint32_t ey = tp.first_active_element[1]; ey < tp.last_active_element[1]; ey++) {
+                    const uint32_t e = ex + ey * lx;
+                    uint32_t ch;
+                    if (g.element_to_channel) {
+                        if (g.element_to_channel[e] < 0) continue;
+                        ch = static_cast<uint32_t>(g.element_to_channel[e]);
+                    } else {
+                        ch = e % call.rf.n_ch;
+                    }
+                    const T xe = G.ex[e], ze = G.ey[e];
+                    const T dxs = xe - x, dzs = ze - z;
+                    if ((dxs * dxs + dzs * dzs) <= aDT) {
+                        const T wxv = window_get(lut, dxs * invx);
+                        const T wzv = window_get(lut, dzs * invz);
+                        const T w = vsqrt(wxv * wzv);
+                        wsum += w;
+                        nadd++;
+                        const T ax = xe - (x + dirx * d);
+                        const T az = ze - (z + dirz * d);
+                        const T ay = diry * d;
+                        T tau = t0 + (vsqrt((ax * ax + az * az) + ay * ay) + d);
+                        tau = tau + off;
+                        // static_cast<uint32_t>(floor(.)) on the GPU saturates
+                        // negative values to 0 (RxSampleBeamformerDelayAndSum.h:87).
+                        const T fl = vfloor(tau);
+                        const uint32_t i = (fl > T(0)) ? static_cast<uint32_t>(fl) : 0u;
+                        const T f = tau - static_cast<T>(i);
+                        const uint64_t base = tx_base + static_cast<uint64_t>(ch) * call.rf.n_s;
+                        if (i < call.rf.n_s - 1) {
+                            const T v0 = rf_at<T>(call.rf, base + i);
+                            const T v1 = rf_at<T>(call.rf, base + i + 1);
+                            sample += w * ((T(1) - f) * v0 + f * v1);
+                        } else if (i < call.rf.n_s && f == T(0)) {
+                            sample += w * rf_at<T>(call.rf, base + i);
+                        }
+                    }
+                }
+            }
+            s_local = (nadd > 0) ? (sample / wsum) * static_cast<T>(nadd) : T(0);
+        }
+        if (call.interp_tx) s_interp += static_cast<T>(static_cast<float>(sc.tx_weights[k])) * s_local;
+        else s_interp += s_local;
+    }
+    return clamp_out(s_interp, call.out_dtype);
+}
+
+template <typename T>
+void das_run(const DasCall& call, const uint32_t* pixels, uint64_t n_pixels, T* out, int nthreads) {
+    const supra_rx_geometry& g = *call.g;
+    const double dt = 1.0 / call.fs;
+    DtGeometry<T> G = to_dt_space<T>(g, dt, call.c);
+    T lut[kEntries];
+    window_lut<T>(call.window, call.wparam, lut);
+    const bool is3d = g.rx_scanline_layout[0] > 1 && g.rx_scanline_layout[1] > 1;
+    const uint32_t n_rx = g.num_rx_scanlines;
+    const uint64_t total = pixels ? n_pixels : static_cast<uint64_t>(n_rx) * g.num_depths;
+    parallel_for(static_cast<int64_t>(total), nthreads, [&](int64_t q) {
+        const uint64_t pix = pixels ? pixels[q] : static_cast<uint64_t>(q);
+        const uint32_t r = static_cast<uint32_t>(pix / n_rx), s = static_cast<uint32_t>(pix % n_rx);
+        out[q] = is3d ? das_pixel_3d<T>(call, G, lut, s, r) : das_pixel_2d<T>(call, G, lut, s, r);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+// Window table in FP32 (precision 32) or FP64 (precision 64).
+int oracle_window_lut(int window_type, float window_parameter, int precision, void* out) {
+    if (!out || window_type < 0 || window_type > 3) return -1;
+    if (precision == 64) window_lut<double>(window_type, window_parameter, static_cast<double*>(out));
+    else window_lut<float>(window_type, window_parameter, static_cast<float*>(out));
+    return 0;
+}
+
+// DAS over all pixels ([depth][scanline]) or the listed pixel indices
+// (r * num_rx + s).  out: float (precision 32) or double (precision 64).
+int oracle_das(const supra_rx_geometry* g, const void* rf, int rf_dtype, uint32_t n_tx, uint32_t n_ch, uint32_t n_s,
+               double fs, double f_number, double c_mm_per_s, int window_type, float window_parameter,
+               int interpolate_transmits, int32_t additional_offset, int test_signal, int out_dtype, int precision,
+               const uint32_t* pixels, uint64_t n_pixels, void* out, int nthreads) {
+    if (!g || !rf || !out || n_tx == 0 || n_ch == 0 || n_s == 0) return -1;
+    for (uint32_t i = 0; i < g->num_rx_scanlines; i++)
+        for (int k = 0; k < 4; k++)
+            if (g->scanlines[i].tx_weights[k] > 0.0 && g->scanlines[i].tx[k].tx_scanline_idx >= n_tx) return -2;
+    DasCall call{g, RfView{rf, rf_dtype, n_tx, n_ch, n_s}, fs, f_number, c_mm_per_s, window_type, window_parameter,
+                 interpolate_transmits, additional_offset, test_signal, out_dtype};
+    if (precision == 64) das_run<double>(call, pixels, n_pixels, static_cast<double*>(out), nthreads);
+    else das_run<float>(call, pixels, n_pixels, static_cast<float*>(out), nthreads);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the supra_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        h = C.CDLL(str(LIB_PATH))
+        h.oracle_window_lut.restype = C.c_int
+        h.oracle_window_lut.argtypes = [C.c_int, C.c_float, C.c_int, C.c_void_p]
+        h.oracle_das.restype = C.c_int
+        h.oracle_das.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_double, C.c_double, C.c_double, C.c_int, C.c_float, C.c_int, C.c_int32,
+                                 C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]
+        _lib = h
+    return _lib
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def window_lut(window_type: int, window_parameter: float, precision: int = 32) -> np.ndarray:
+    out = np.zeros(64, dtype=np.float32 if precision == 32 else np.float64)
+    assert lib().oracle_window_lut(window_type, window_parameter, precision, out.ctypes.data) == 0
+    return out
+
+
+def das(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_per_s: float,
+        window_type: int, window_parameter: float = 0.0, interpolate_transmits: bool = False,
+        additional_offset: int = 0, test_signal: bool = False, out_int16: bool = False,
+        precision: int = 32, pixels: np.ndarray | None = None, threads: int | None = None) -> np.ndarray:
+    """Oracle DAS image [num_depths, num_rx] (or the values at `pixels`,
+    flat indices depth * num_rx + scanline).  `geom` is a RxGeometry."""
+    rf = np.ascontiguousarray(rf)
+    assert rf.ndim == 3 and rf.dtype in (np.int16, np.float32)
+    gc = geom.to_c()
+    dt = np.float32 if precision == 32 else np.float64
+    if pixels is None:
+        out = np.zeros((geom.num_depths, geom.num_rx), dtype=dt)
+        pix_ptr, npix = None, 0
+    else:
+        pixels = np.ascontiguousarray(pixels, dtype=np.uint32)
+        out = np.zeros(pixels.shape[0], dtype=dt)
+        pix_ptr, npix = pixels.ctypes.data, pixels.shape[0]
+    code = lib().oracle_das(C.addressof(gc), rf.ctypes.data, 1 if rf.dtype == np.int16 else 2,
+                            rf.shape[0], rf.shape[1], rf.shape[2], fs_hz, f_number, speed_of_sound_mm_per_s,
+                            window_type, window_parameter, int(interpolate_transmits), additional_offset,
+                            int(test_signal), 1 if out_int16 else 2, precision, pix_ptr, npix,
+                            out.ctypes.data, threads or default_threads())
+    if code != 0:
+        raise ValueError(f"oracle_das rejected its arguments ({code})")
+    return out

@@ -1,0 +1,76 @@
+"""Build the HIP beamformer library in-tree (supra_amd/libsupra_bf.so).
+
+Host C++ sources are compiled with g++ and the .hip sources with hipcc for
+gfx950, then linked with hipcc.  Floating-point contraction is disabled
+everywhere: the kernels reproduce the host restatement of the reference's
+single-precision arithmetic bit for bit, which an implicit FMA would break.
+Packed FP32 instructions are disabled for the device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libsupra_bf.so"
+
+ARCH = os.environ.get("SUPRA_OFFLOAD_ARCH", "gfx950")
+
+HOST_SOURCES = ["geometry.cpp", "bf_host.cpp", "status.cpp"]
+HIP_SOURCES = ["bf_context.hip", "das_kernels.hip", "synth.hip"]
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the HIP beamformer cannot be built")
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every source and link libsupra_bf.so; returns its path."""
+    BUILD.mkdir(exist_ok=True)
+    hipcc = _hipcc()
+    jobs = []
+    objs = []
+    for src in HOST_SOURCES:
+        obj = BUILD / (src + ".o")
+        objs.append(obj)
+        jobs.append(["g++", *COMMON, "-c", str(CSRC / src), "-o", str(obj)])
+    for src in HIP_SOURCES:
+        obj = BUILD / (src + ".o")
+        objs.append(obj)
+        jobs.append([hipcc, *COMMON, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+    workers = min(len(jobs), int(os.environ.get("MAX_JOBS", "8")))
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        list(ex.map(lambda c: _run(c, verbose), jobs))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,310 @@
+// Engine context and the DAS entry points of the C ABI.
+//
+// Stands in for RxBeamformerCuda (src/SupraLib/Beamformer/RxBeamformerCuda.cu:24-107,
+// 539-663): the constructor keeps the receive geometry, and every call whose
+// sampling interval or speed of sound differs from the previous one rebuilds
+// the dt-space tables.  Unlike the reference, which rescales its float
+// buffers by old/new factors on each change (:60-102, so repeated changes
+// accumulate rounding), the tables are always rebuilt from the millimetre
+// geometry with the reference's first-call factors 1/(c*dt) and 1/dt.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "bf_internal.hpp"
+#include "das_launch.hpp"
+#include "status.hpp"
+#include "supra_bf.h"
+
+using namespace supra_amd;
+
+struct supra_bf_ctx {
+    int device = 0;
+    uint32_t n_rx = 0, rx_layout[2] = {0, 0}, n_depths = 0, n_el = 0, el_layout[2] = {0, 0};
+    bool is3d = false;
+    std::vector<supra_scanline_rx> scan_mm;
+    std::vector<float> depths_mm, ex_mm, ey_mm;
+    std::vector<int32_t> e2c;
+    double geom_c = 0, depth_mm = 0;
+    uint32_t max_channel_in_map = 0;
+
+    ScanlineDT* d_scans = nullptr;
+    float* d_depths = nullptr;
+    float* d_ex = nullptr;
+    float* d_ey = nullptr;
+    int32_t* d_e2c = nullptr;
+
+    std::vector<ScanlineDT> h_scans;
+    std::vector<float> h_depths, h_ex, h_ey;
+    double cached_dt = 0, cached_c = 0;
+    std::mutex mu;
+
+    ~supra_bf_ctx() {
+        (void)hipFree(d_scans);
+        (void)hipFree(d_depths);
+        (void)hipFree(d_ex);
+        (void)hipFree(d_ey);
+        (void)hipFree(d_e2c);
+    }
+};
+
+namespace {
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(SUPRA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// RxBeamformerCuda::convertToDtSpace (RxBeamformerCuda.cu:58-107), first-call
+// factors, from the millimetre geometry; then the float casts of the kernels.
+void build_dt_tables(supra_bf_ctx* ctx, double dt, double c) {
+    const double factor = 1 / 1.0 / (c * dt);
+    const double factor_time = 1 / 1.0 / dt;
+    ctx->h_scans.resize(ctx->n_rx);
+    for (uint32_t i = 0; i < ctx->n_rx; i++) {
+        const supra_scanline_rx& s = ctx->scan_mm[i];
+        ScanlineDT& o = ctx->h_scans[i];
+        std::memset(&o, 0, sizeof(o));
+        o.x = static_cast<float>(s.position[0] * factor);
+        o.z = static_cast<float>(s.position[2] * factor);
+        o.dir_x = static_cast<float>(s.direction[0]);
+        o.dir_y = static_cast<float>(s.direction[1]);
+        o.dir_z = static_cast<float>(s.direction[2]);
+        o.max_ed_x = static_cast<float>(s.max_element_distance[0] * factor);
+        o.max_ed_y = static_cast<float>(s.max_element_distance[1] * factor);
+        scanline_selection(s, &o.sel, &o.active_mask, o.txw);
+        for (int k = 0; k < 4; k++) {
+            o.tx[k].first_x = s.tx[k].first_active_element[0];
+            o.tx[k].first_y = s.tx[k].first_active_element[1];
+            o.tx[k].last_x = s.tx[k].last_active_element[0];
+            o.tx[k].last_y = s.tx[k].last_active_element[1];
+            o.tx[k].tx_idx = s.tx[k].tx_scanline_idx;
+            o.tx[k].t0 = static_cast<float>(s.tx[k].initial_delay * factor_time);
+        }
+    }
+    ctx->h_depths.resize(ctx->n_depths);
+    for (uint32_t i = 0; i < ctx->n_depths; i++)
+        ctx->h_depths[i] = static_cast<float>(ctx->depths_mm[i] * factor);
+    ctx->h_ex.resize(ctx->n_el);
+    ctx->h_ey.resize(ctx->n_el);
+    for (uint32_t i = 0; i < ctx->n_el; i++) {
+        ctx->h_ex[i] = static_cast<float>(ctx->ex_mm[i] * factor);
+        ctx->h_ey[i] = static_cast<float>(ctx->ey_mm[i] * factor);
+    }
+}
+
+int ensure_dt_tables(supra_bf_ctx* ctx, double dt, double c, hipStream_t stream) {
+    if (ctx->cached_dt == dt && ctx->cached_c == c) return SUPRA_OK;
+    build_dt_tables(ctx, dt, c);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(ctx->d_scans, ctx->h_scans.data(), ctx->n_rx * sizeof(ScanlineDT), hipMemcpyHostToDevice,
+                            stream)) != hipSuccess)
+        return hip_fail(e, "upload scanlines");
+    if ((e = hipMemcpyAsync(ctx->d_depths, ctx->h_depths.data(), ctx->n_depths * sizeof(float), hipMemcpyHostToDevice,
+                            stream)) != hipSuccess)
+        return hip_fail(e, "upload depths");
+    if ((e = hipMemcpyAsync(ctx->d_ex, ctx->h_ex.data(), ctx->n_el * sizeof(float), hipMemcpyHostToDevice, stream)) !=
+        hipSuccess)
+        return hip_fail(e, "upload element x");
+    if ((e = hipMemcpyAsync(ctx->d_ey, ctx->h_ey.data(), ctx->n_el * sizeof(float), hipMemcpyHostToDevice, stream)) !=
+        hipSuccess)
+        return hip_fail(e, "upload element y");
+    // The host tables are rewritten on the next change: wait for the copies.
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "geometry upload");
+    ctx->cached_dt = dt;
+    ctx->cached_c = c;
+    return SUPRA_OK;
+}
+
+int check_device(const supra_bf_ctx* ctx) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev != ctx->device)
+        return fail(SUPRA_ERR_INVALID_ARG, "context was created on device %d, current device is %d", ctx->device, dev);
+    return SUPRA_OK;
+}
+
+}  // namespace
+
+extern "C" int supra_bf_create(const supra_rx_geometry* g, supra_bf_ctx** out_ctx) {
+    clear_error();
+    if (!g || !out_ctx) return fail(SUPRA_ERR_INVALID_ARG, "create: null argument");
+    *out_ctx = nullptr;
+    if (!g->scanlines || !g->depths || !g->element_x || !g->element_y)
+        return fail(SUPRA_ERR_INVALID_ARG, "create: geometry arrays must be non-null");
+    if (g->num_rx_scanlines == 0 || g->num_depths == 0 || g->num_elements == 0)
+        return fail(SUPRA_ERR_INVALID_ARG, "create: empty geometry");
+    if (static_cast<uint64_t>(g->rx_scanline_layout[0]) * g->rx_scanline_layout[1] != g->num_rx_scanlines)
+        return fail(SUPRA_ERR_INVALID_ARG, "create: rx_scanline_layout does not multiply to num_rx_scanlines");
+    if (static_cast<uint64_t>(g->element_layout[0]) * g->element_layout[1] != g->num_elements)
+        return fail(SUPRA_ERR_INVALID_ARG, "create: element_layout does not multiply to num_elements");
+    if (g->num_rx_scanlines > (1u << 24) || g->num_depths > (1u << 24))
+        return fail(SUPRA_ERR_TOO_LARGE, "create: geometry too large");
+
+    supra_bf_ctx* ctx = new (std::nothrow) supra_bf_ctx();
+    if (!ctx) return fail(SUPRA_ERR_INVALID_ARG, "create: out of host memory");
+    ctx->n_rx = g->num_rx_scanlines;
+    ctx->rx_layout[0] = g->rx_scanline_layout[0];
+    ctx->rx_layout[1] = g->rx_scanline_layout[1];
+    ctx->n_depths = g->num_depths;
+    ctx->n_el = g->num_elements;
+    ctx->el_layout[0] = g->element_layout[0];
+    ctx->el_layout[1] = g->element_layout[1];
+    ctx->is3d = g->rx_scanline_layout[0] > 1 && g->rx_scanline_layout[1] > 1;
+    ctx->scan_mm.assign(g->scanlines, g->scanlines + g->num_rx_scanlines);
+    ctx->depths_mm.assign(g->depths, g->depths + g->num_depths);
+    ctx->ex_mm.assign(g->element_x, g->element_x + g->num_elements);
+    ctx->ey_mm.assign(g->element_y, g->element_y + g->num_elements);
+    ctx->geom_c = g->speed_of_sound_mm_per_s;
+    ctx->depth_mm = g->depth;
+    if (g->element_to_channel) {
+        ctx->e2c.assign(g->element_to_channel, g->element_to_channel + g->num_elements);
+        for (int32_t m : ctx->e2c) {
+            if (m < SUPRA_ELEMENT_NOT_CONNECTED) {
+                delete ctx;
+                return fail(SUPRA_ERR_INVALID_ARG, "create: element_to_channel entries must be >= -1");
+            }
+            if (m >= 0) ctx->max_channel_in_map = std::max(ctx->max_channel_in_map, static_cast<uint32_t>(m));
+        }
+    }
+    // Aperture corners must address existing elements (the reference reads
+    // element tables unchecked).
+    for (uint32_t i = 0; i < ctx->n_rx; i++) {
+        for (int k = 0; k < 4; k++) {
+            const supra_tx_params& t = ctx->scan_mm[i].tx[k];
+            bool bad = ctx->is3d ? (t.last_active_element[0] > ctx->el_layout[0] ||
+                                    t.last_active_element[1] > ctx->el_layout[1])
+                                 : (t.last_active_element[0] > ctx->n_el);
+            if (bad) {
+                delete ctx;
+                return fail(SUPRA_ERR_INVALID_ARG, "create: scanline %u transmit %d aperture exceeds the array", i, k);
+            }
+        }
+    }
+    hipError_t e = hipGetDevice(&ctx->device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(SUPRA_ERR_NO_DEVICE, "create: no HIP device: %s", hipGetErrorString(e));
+    }
+    if ((e = hipMalloc(&ctx->d_scans, ctx->n_rx * sizeof(ScanlineDT))) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_depths, ctx->n_depths * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_ex, ctx->n_el * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_ey, ctx->n_el * sizeof(float))) != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "create: hipMalloc");
+    }
+    if (!ctx->e2c.empty()) {
+        if ((e = hipMalloc(&ctx->d_e2c, ctx->n_el * sizeof(int32_t))) != hipSuccess ||
+            (e = hipMemcpy(ctx->d_e2c, ctx->e2c.data(), ctx->n_el * sizeof(int32_t), hipMemcpyHostToDevice)) !=
+                hipSuccess) {
+            delete ctx;
+            return hip_fail(e, "create: element map upload");
+        }
+    }
+    *out_ctx = ctx;
+    return SUPRA_OK;
+}
+
+extern "C" void supra_bf_destroy(supra_bf_ctx* ctx) { delete ctx; }
+
+extern "C" int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer, uint32_t num_frames,
+                                    const void* rf, int32_t rf_dtype, uint32_t num_tx, uint32_t num_channels,
+                                    uint32_t num_samples, double sampling_frequency_hz, double f_number,
+                                    double speed_of_sound_mm_per_s, int32_t window_type, float window_parameter,
+                                    int32_t interpolate_transmits, int32_t additional_offset, void* out,
+                                    int32_t out_dtype, void* stream) {
+    clear_error();
+    if (!ctx || !rf || !out) return fail(SUPRA_ERR_INVALID_ARG, "das: null argument");
+    if (num_frames == 0) return SUPRA_OK;
+    if ((rf_dtype != SUPRA_INT16 && rf_dtype != SUPRA_FLOAT32) ||
+        (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
+        return fail(SUPRA_ERR_UNSUPPORTED, "das: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+    if (sample_beamformer != SUPRA_BF_DELAY_AND_SUM && sample_beamformer != SUPRA_BF_TEST_SIGNAL)
+        return fail(SUPRA_ERR_UNSUPPORTED, "das: sample beamformer %d is not provided (DelayAndSum=0, TestSignal=3)",
+                    sample_beamformer);
+    if (window_type < SUPRA_WINDOW_RECTANGULAR || window_type > SUPRA_WINDOW_GAUSS)
+        return fail(SUPRA_ERR_INVALID_ARG, "das: unknown window type %d", window_type);
+    if (num_tx == 0 || num_channels == 0 || num_samples == 0)
+        return fail(SUPRA_ERR_INVALID_ARG, "das: empty RF (num_tx=%u num_channels=%u num_samples=%u)", num_tx,
+                    num_channels, num_samples);
+    if (static_cast<uint64_t>(num_channels) * num_samples >= (1ull << 32))
+        return fail(SUPRA_ERR_TOO_LARGE, "das: one transmit's RF block must hold fewer than 2^32 samples");
+    if (!(sampling_frequency_hz > 0) || !(speed_of_sound_mm_per_s > 0) || !(f_number > 0) ||
+        !std::isfinite(sampling_frequency_hz) || !std::isfinite(speed_of_sound_mm_per_s) || !std::isfinite(f_number))
+        return fail(SUPRA_ERR_INVALID_ARG, "das: sampling frequency, speed of sound and f-number must be positive");
+    if (!ctx->e2c.empty() && ctx->max_channel_in_map >= num_channels)
+        return fail(SUPRA_ERR_INVALID_ARG, "das: element map addresses channel %u of %u", ctx->max_channel_in_map,
+                    num_channels);
+    // Transmit indices the kernels will read (RxBeamformerCuda.cu:299-303
+    // leaves such pixels unwritten; here the call is rejected instead).
+    for (uint32_t i = 0; i < ctx->n_rx; i++) {
+        const supra_scanline_rx& s = ctx->scan_mm[i];
+        int32_t sel, mask;
+        float w[4];
+        scanline_selection(s, &sel, &mask, w);
+        for (int k = 0; k < 4; k++) {
+            bool used = interpolate_transmits ? ((mask >> k) & 1) : (k == sel);
+            if (used && s.tx[k].tx_scanline_idx >= num_tx)
+                return fail(SUPRA_ERR_INVALID_ARG, "das: scanline %u uses transmit %u of %u", i,
+                            static_cast<unsigned>(s.tx[k].tx_scanline_idx), num_tx);
+        }
+    }
+    const uint64_t tile = das_depth_tile();
+    const uint64_t blocks = ((ctx->n_depths + tile - 1) / tile) * ctx->n_rx * static_cast<uint64_t>(num_frames);
+    if (blocks >= (1ull << 31)) return fail(SUPRA_ERR_TOO_LARGE, "das: too many frames for one launch");
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int st = check_device(ctx);
+    if (st != SUPRA_OK) return st;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const double dt = 1.0 / sampling_frequency_hz;
+    st = ensure_dt_tables(ctx, dt, speed_of_sound_mm_per_s, hs);
+    if (st != SUPRA_OK) return st;
+
+    DasArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.rf = rf;
+    a.out = out;
+    a.scans = ctx->d_scans;
+    a.depths = ctx->d_depths;
+    a.ex = ctx->d_ex;
+    a.ey = ctx->d_ey;
+    a.e2c = ctx->d_e2c;
+    a.rf_frame_stride = static_cast<uint64_t>(num_tx) * num_channels * num_samples;
+    a.out_frame_stride = static_cast<uint64_t>(ctx->n_rx) * ctx->n_depths;
+    a.n_rx = ctx->n_rx;
+    a.n_depths = ctx->n_depths;
+    a.n_tx = num_tx;
+    a.n_ch = num_channels;
+    a.n_s = num_samples;
+    a.n_el = ctx->n_el;
+    a.layout_x = ctx->el_layout[0];
+    a.layout_y = ctx->el_layout[1];
+    a.n_frames = num_frames;
+    a.depth_tiles = static_cast<uint32_t>((ctx->n_depths + tile - 1) / tile);
+    a.F = static_cast<float>(f_number);
+    a.dt = static_cast<float>(dt);
+    a.c = static_cast<float>(speed_of_sound_mm_per_s);
+    a.offset = static_cast<float>(additional_offset);
+    build_window_lut(window_type, window_parameter, a.lut);
+
+    hipError_t e = launch_das(a, rf_dtype, out_dtype, ctx->is3d, sample_beamformer == SUPRA_BF_TEST_SIGNAL,
+                              interpolate_transmits != 0, ctx->d_e2c != nullptr, hs);
+    if (e != hipSuccess) return hip_fail(e, "das: launch");
+    return SUPRA_OK;
+}
+
+extern "C" int supra_das_run(supra_bf_ctx* ctx, int32_t sample_beamformer, const void* rf, int32_t rf_dtype,
+                             uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                             double sampling_frequency_hz, double f_number, double speed_of_sound_mm_per_s,
+                             int32_t window_type, float window_parameter, int32_t interpolate_transmits,
+                             int32_t additional_offset, void* out, int32_t out_dtype, void* stream) {
+    return supra_das_run_frames(ctx, sample_beamformer, 1, rf, rf_dtype, num_tx, num_channels, num_samples,
+                                sampling_frequency_hz, f_number, speed_of_sound_mm_per_s, window_type,
+                                window_parameter, interpolate_transmits, additional_offset, out, out_dtype, stream);
+}

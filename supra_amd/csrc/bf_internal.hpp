@@ -1,0 +1,66 @@
+// Internal types shared by the host engine and the HIP kernels.
+//
+// The device-side scanline table is the reference's ScanlineRxParameters3D
+// (src/SupraLib/USImageProperties.h:43-95) already converted to dt-space and
+// to the single-precision values the kernels read
+// (RxBeamformerCuda.cu:58-107 for the conversion, :164-175 and :261-270 for
+// the float casts), plus the transmit selection that the reference kernels
+// recompute per pixel (:274-303), done once on the host with the same
+// double/float comparison.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "supra_bf.h"
+
+namespace supra_amd {
+
+constexpr int kWindowEntries = SUPRA_WINDOW_NUM_ENTRIES;
+
+struct TxDT {
+    uint16_t first_x, first_y, last_x, last_y;   // loop is first <= e < last
+    uint32_t tx_idx;
+    float t0;                                    // initialDelay in samples
+};
+
+struct ScanlineDT {
+    float x, z;                 // position.x, position.z (dt-space)
+    float dir_x, dir_y, dir_z;
+    float max_ed_x, max_ed_y;   // maxElementDistance (dt-space)
+    int32_t sel;                // transmit used without interpolation; -1: none (weight <= 0)
+    float txw[4];               // (float) txWeights[k]
+    int32_t active_mask;        // bit k: txWeights[k] > 0.0
+    int32_t pad[3];
+    TxDT tx[4];
+};
+static_assert(sizeof(TxDT) == 16, "TxDT layout");
+static_assert(sizeof(ScanlineDT) == 128, "ScanlineDT layout");
+
+// Everything a DAS launch needs, passed by value as the kernel argument.
+struct DasArgs {
+    const void* rf;
+    void* out;
+    const ScanlineDT* scans;
+    const float* depths;        // dt-space
+    const float* ex;            // element x (dt-space)
+    const float* ey;            // element y (dt-space), the kernels' "z"
+    const int32_t* e2c;         // element -> channel, or nullptr
+    uint64_t rf_frame_stride;   // elements between frames
+    uint64_t out_frame_stride;
+    uint32_t n_rx, n_depths, n_tx, n_ch, n_s, n_el;
+    uint32_t layout_x, layout_y;
+    uint32_t n_frames;
+    uint32_t depth_tiles;       // ceil(n_depths / tile)
+    float F, dt, c;             // fNumber, 1/fs, speed of sound [mm/s] as floats
+    float offset;               // (float) additionalOffset
+    int32_t max_aperture;       // largest (last-first) product over the used transmits
+    int32_t pad0;
+    float lut[kWindowEntries];
+};
+
+// Host helpers (bf_host.cpp).
+void build_window_lut(int window_type, float window_parameter, float* lut);
+bool scanline_selection(const supra_scanline_rx& s, int32_t* sel, int32_t* active_mask, float txw[4]);
+
+}  // namespace supra_amd
