@@ -160,6 +160,9 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) so that "
+                         "FETCH_SIZE/WRITE_SIZE counters of a profile can be scaled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this beamformer runs on one GPU; --gpus must be 1 "
@@ -170,6 +173,13 @@ def main() -> int:
         print(json.dumps({"error": "no HIP device"}))
         return 1
 
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     res = bench_3d(args, torch)
     algo_gbs = res["bytes"] / (res["ev_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(algo_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

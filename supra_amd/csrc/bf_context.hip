@@ -42,6 +42,7 @@ struct supra_bf_ctx {
     std::vector<ScanlineDT> h_scans;
     std::vector<float> h_depths, h_ex, h_ey;
     double cached_dt = 0, cached_c = 0;
+    bool far_ok = false;   // dt-space depths are >= 0 and non-decreasing
     std::mutex mu;
 
     ~supra_bf_ctx() {
@@ -64,6 +65,20 @@ int hip_fail(hipError_t e, const char* what) {
 void build_dt_tables(supra_bf_ctx* ctx, double dt, double c) {
     const double factor = 1 / 1.0 / (c * dt);
     const double factor_time = 1 / 1.0 / dt;
+    ctx->h_depths.resize(ctx->n_depths);
+    ctx->far_ok = true;
+    for (uint32_t i = 0; i < ctx->n_depths; i++) {
+        ctx->h_depths[i] = static_cast<float>(ctx->depths_mm[i] * factor);
+        if (!(ctx->h_depths[i] >= 0.0f) || (i > 0 && !(ctx->h_depths[i] >= ctx->h_depths[i - 1])))
+            ctx->far_ok = false;
+    }
+    ctx->h_ex.resize(ctx->n_el);
+    ctx->h_ey.resize(ctx->n_el);
+    for (uint32_t i = 0; i < ctx->n_el; i++) {
+        ctx->h_ex[i] = static_cast<float>(ctx->ex_mm[i] * factor);
+        ctx->h_ey[i] = static_cast<float>(ctx->ey_mm[i] * factor);
+    }
+    auto connected = [&](uint32_t e) { return ctx->e2c.empty() || ctx->e2c[e] >= 0; };
     ctx->h_scans.resize(ctx->n_rx);
     for (uint32_t i = 0; i < ctx->n_rx; i++) {
         const supra_scanline_rx& s = ctx->scan_mm[i];
@@ -78,22 +93,30 @@ void build_dt_tables(supra_bf_ctx* ctx, double dt, double c) {
         o.max_ed_y = static_cast<float>(s.max_element_distance[1] * factor);
         scanline_selection(s, &o.sel, &o.active_mask, o.txw);
         for (int k = 0; k < 4; k++) {
-            o.tx[k].first_x = s.tx[k].first_active_element[0];
-            o.tx[k].first_y = s.tx[k].first_active_element[1];
-            o.tx[k].last_x = s.tx[k].last_active_element[0];
-            o.tx[k].last_y = s.tx[k].last_active_element[1];
-            o.tx[k].tx_idx = s.tx[k].tx_scanline_idx;
-            o.tx[k].t0 = static_cast<float>(s.tx[k].initial_delay * factor_time);
+            TxDT& t = o.tx[k];
+            t.first_x = s.tx[k].first_active_element[0];
+            t.first_y = s.tx[k].first_active_element[1];
+            t.last_x = s.tx[k].last_active_element[0];
+            t.last_y = s.tx[k].last_active_element[1];
+            t.tx_idx = s.tx[k].tx_scanline_idx;
+            t.t0 = static_cast<float>(s.tx[k].initial_delay * factor_time);
+            // Same single-precision operations as the kernels' aperture test.
+            float thr = 0.0f;
+            if (ctx->is3d) {
+                for (uint32_t ex = t.first_x; ex < t.last_x; ex++)
+                    for (uint32_t ey = t.first_y; ey < t.last_y; ey++) {
+                        const uint32_t e = ex + ey * ctx->el_layout[0];
+                        if (!connected(e)) continue;
+                        const float dx = ctx->h_ex[e] - o.x, dz = ctx->h_ey[e] - o.z;
+                        thr = std::max(thr, dx * dx + dz * dz);
+                    }
+            } else {
+                for (uint32_t e = t.first_x; e < t.last_x; e++)
+                    if (connected(e)) thr = std::max(thr, std::fabs(ctx->h_ex[e] - o.x));
+                thr = std::max(thr, o.max_ed_x);
+            }
+            t.far_thr = thr;
         }
-    }
-    ctx->h_depths.resize(ctx->n_depths);
-    for (uint32_t i = 0; i < ctx->n_depths; i++)
-        ctx->h_depths[i] = static_cast<float>(ctx->depths_mm[i] * factor);
-    ctx->h_ex.resize(ctx->n_el);
-    ctx->h_ey.resize(ctx->n_el);
-    for (uint32_t i = 0; i < ctx->n_el; i++) {
-        ctx->h_ex[i] = static_cast<float>(ctx->ex_mm[i] * factor);
-        ctx->h_ey[i] = static_cast<float>(ctx->ey_mm[i] * factor);
     }
 }
 
@@ -232,8 +255,8 @@ extern "C" int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer
     if (num_tx == 0 || num_channels == 0 || num_samples == 0)
         return fail(SUPRA_ERR_INVALID_ARG, "das: empty RF (num_tx=%u num_channels=%u num_samples=%u)", num_tx,
                     num_channels, num_samples);
-    if (static_cast<uint64_t>(num_channels) * num_samples >= (1ull << 32))
-        return fail(SUPRA_ERR_TOO_LARGE, "das: one transmit's RF block must hold fewer than 2^32 samples");
+    if (static_cast<uint64_t>(num_channels) * num_samples * (rf_dtype == SUPRA_INT16 ? 2 : 4) >= (1ull << 31))
+        return fail(SUPRA_ERR_TOO_LARGE, "das: one transmit's RF block must be smaller than 2 GiB");
     if (!(sampling_frequency_hz > 0) || !(speed_of_sound_mm_per_s > 0) || !(f_number > 0) ||
         !std::isfinite(sampling_frequency_hz) || !std::isfinite(speed_of_sound_mm_per_s) || !std::isfinite(f_number))
         return fail(SUPRA_ERR_INVALID_ARG, "das: sampling frequency, speed of sound and f-number must be positive");
@@ -291,6 +314,7 @@ extern "C" int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer
     a.dt = static_cast<float>(dt);
     a.c = static_cast<float>(speed_of_sound_mm_per_s);
     a.offset = static_cast<float>(additional_offset);
+    a.far_ok = ctx->far_ok ? 1 : 0;
     build_window_lut(window_type, window_parameter, a.lut);
 
     hipError_t e = launch_das(a, rf_dtype, out_dtype, ctx->is3d, sample_beamformer == SUPRA_BF_TEST_SIGNAL,

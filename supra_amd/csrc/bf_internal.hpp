@@ -22,6 +22,13 @@ struct TxDT {
     uint16_t first_x, first_y, last_x, last_y;   // loop is first <= e < last
     uint32_t tx_idx;
     float t0;                                    // initialDelay in samples
+    // Far-field threshold on the aperture half-width aDT: at depths with
+    // aDT >= far_thr every connected aperture element is inside the F-number
+    // aperture and the window argument no longer depends on depth.
+    //   2D: max(maxElementDistance.x, max |x_e - x_s|)
+    //   3D: max (x_e - x_s)^2 + (z_e - z_s)^2   (aDT is squared in 3D)
+    float far_thr;
+    uint32_t pad[3];
 };
 
 struct ScanlineDT {
@@ -34,8 +41,8 @@ struct ScanlineDT {
     int32_t pad[3];
     TxDT tx[4];
 };
-static_assert(sizeof(TxDT) == 16, "TxDT layout");
-static_assert(sizeof(ScanlineDT) == 128, "ScanlineDT layout");
+static_assert(sizeof(TxDT) == 32, "TxDT layout");
+static_assert(sizeof(ScanlineDT) == 192, "ScanlineDT layout");
 
 // Everything a DAS launch needs, passed by value as the kernel argument.
 struct DasArgs {
@@ -54,8 +61,8 @@ struct DasArgs {
     uint32_t depth_tiles;       // ceil(n_depths / tile)
     float F, dt, c;             // fNumber, 1/fs, speed of sound [mm/s] as floats
     float offset;               // (float) additionalOffset
-    int32_t max_aperture;       // largest (last-first) product over the used transmits
-    int32_t pad0;
+    int32_t max_aperture;       // unused (0)
+    int32_t far_ok;             // depth table is non-negative and non-decreasing
     float lut[kWindowEntries];
 };
 

@@ -15,14 +15,25 @@
 // Launch layout (MI355X): one workgroup of 256 lanes (4 wave64) per
 // (frame, rx scanline, tile of 256 depths).  Lanes map to consecutive
 // depths, so the two RF gathers of one element hit a contiguous window of
-// ~2x256 samples of one channel (coalesced into a few cache lines).  The
-// per-element quantities that do not depend on depth (element position
-// relative to the scanline, channel offset, their squares) are computed once
-// per workgroup into LDS and broadcast to all lanes.
+// ~2x256 samples of one channel.  Per-element quantities that do not depend
+// on depth are computed once per workgroup into LDS and broadcast.
+//
+// The kernels are VALU-bound (a correctly rounded sqrt and a linear
+// interpolation per element and pixel), so the work per element is trimmed:
+//  * far-field tiles: once the F-number aperture aDT at the tile's first
+//    depth covers every aperture element and exceeds maxElementDistance
+//    (aDT grows monotonically with depth), membership is always true and the
+//    window argument no longer depends on depth, so each element's weight is
+//    looked up once per workgroup;
+//  * RF gathers are raw buffer loads (32-bit offsets, hardware range check
+//    returns 0 beyond the transmit's block), issued unconditionally so that
+//    the out-of-range cases of the reference become selects, not branches,
+//    and several elements' loads are in flight at once.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 
 #include "bf_internal.hpp"
@@ -34,8 +45,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kChunk2D = 512;     // aperture elements staged per LDS pass (2D)
-constexpr int kChunk3D = 1024;    // aperture elements staged per LDS pass (3D)
+constexpr int kChunk3D = 512;     // aperture elements staged per LDS pass (3D)
+constexpr int kW2Stride = kWindowEntries + 1;   // padded rows: spread LDS banks
 constexpr uint32_t kNotConnected = 0xffffffffu;
+constexpr int kBufferDword3 = 0x00020000;       // gfx9 raw buffer descriptor word 3
 
 // clampCast (src/SupraLib/utilities/cudaUtility.h:111-147): fmaxf/fminf keep
 // the non-NaN operand, so a NaN input lands on the lower limit.
@@ -47,88 +60,197 @@ template <> __device__ inline int16_t clamp_cast<int16_t>(float x) {
     return static_cast<int16_t>(fminf(fmaxf(x, -32767.0f), 32767.0f));
 }
 
-template <typename T> __device__ inline float to_f(T v) { return static_cast<float>(v); }
-
-// WindowFunctionGpu::get (WindowFunction.h:58-64) on an LDS copy of the LUT.
+// WindowFunctionGpu::get (WindowFunction.h:58-64): roundf of a value in
+// [0, 63], computed as truncation plus a half-up step (identical to roundf
+// for non-negative arguments).
 __device__ inline int window_index(float rel) {
-    float cl = fminf(fmaxf(rel, -1.0f), 1.0f);
-    return static_cast<int>(static_cast<uint32_t>(roundf(31.5f * (cl + 1.0f))));
+    const float x = 31.5f * (fminf(fmaxf(rel, -1.0f), 1.0f) + 1.0f);
+    const uint32_t t = static_cast<uint32_t>(x);
+    return static_cast<int>(t + ((x - static_cast<float>(t)) >= 0.5f ? 1u : 0u));
 }
 
 // Correctly rounded single-precision square root for x == 0 or normal x
 // (every argument here is a sum of squares of dt-space lengths, so it is
 // either exactly zero or far above 2^-96).  v_sqrt_f32 is within one ulp;
 // one residual test on each neighbour picks the correctly rounded value
-// (the same correction LLVM emits for sqrtf, without the denormal rescale
-// and the class test that this domain does not need).
+// (the correction LLVM emits for sqrtf, without the denormal rescale and the
+// class test that this domain does not need).
 __device__ inline float sqrt_rn(float x) {
     float s = __builtin_amdgcn_sqrtf(x);
-    uint32_t si = __float_as_uint(s);
-    float down = __uint_as_float(si - 1u);
-    float up = __uint_as_float(si + 1u);
-    float r_down = fmaf(-down, s, x);
-    float r_up = fmaf(-up, s, x);
+    const uint32_t si = __float_as_uint(s);
+    const float down = __uint_as_float(si - 1u);
+    const float up = __uint_as_float(si + 1u);
+    const float r_down = fmaf(-down, s, x);
+    const float r_up = fmaf(-up, s, x);
     s = (r_down <= 0.0f) ? down : s;
     s = (r_up > 0.0f) ? up : s;
     return s;
+}
+
+// Workgroup-uniform tables are read through the constant address space so
+// that they arrive in SGPRs (scalar loads); the buffer descriptor built from
+// them is then uniform and the gathers need no per-lane descriptor loop.
+typedef const __attribute__((address_space(4))) uint32_t* UniformWords;
+
+// Field accessors of one ScanlineDT record through scalar loads (no copy of
+// the record into registers or private memory; transmit k may be dynamic).
+struct ScanView {
+    UniformWords w;
+    __device__ float f(size_t byte_offset) const { return __uint_as_float(w[byte_offset / 4]); }
+    __device__ uint32_t u(size_t byte_offset) const { return w[byte_offset / 4]; }
+    __device__ float x() const { return f(offsetof(ScanlineDT, x)); }
+    __device__ float z() const { return f(offsetof(ScanlineDT, z)); }
+    __device__ float dir_x() const { return f(offsetof(ScanlineDT, dir_x)); }
+    __device__ float dir_y() const { return f(offsetof(ScanlineDT, dir_y)); }
+    __device__ float dir_z() const { return f(offsetof(ScanlineDT, dir_z)); }
+    __device__ float max_ed_x() const { return f(offsetof(ScanlineDT, max_ed_x)); }
+    __device__ float max_ed_y() const { return f(offsetof(ScanlineDT, max_ed_y)); }
+    __device__ int32_t sel() const { return static_cast<int32_t>(u(offsetof(ScanlineDT, sel))); }
+    __device__ int32_t active_mask() const { return static_cast<int32_t>(u(offsetof(ScanlineDT, active_mask))); }
+    __device__ float txw(int k) const { return __uint_as_float(w[offsetof(ScanlineDT, txw) / 4 + k]); }
+    __device__ TxDT tx(int k) const {
+        const size_t b = (offsetof(ScanlineDT, tx) + static_cast<size_t>(k) * sizeof(TxDT)) / 4;
+        TxDT t;
+        const uint32_t w0 = w[b], w1 = w[b + 1];
+        t.first_x = static_cast<uint16_t>(w0 & 0xffffu);
+        t.first_y = static_cast<uint16_t>(w0 >> 16);
+        t.last_x = static_cast<uint16_t>(w1 & 0xffffu);
+        t.last_y = static_cast<uint16_t>(w1 >> 16);
+        t.tx_idx = w[b + 2];
+        t.t0 = __uint_as_float(w[b + 3]);
+        t.far_thr = __uint_as_float(w[b + 4]);
+        return t;
+    }
+};
+
+__device__ inline ScanView scan_view(const ScanlineDT* table, uint32_t s) {
+    return ScanView{(UniformWords)(table + s)};
+}
+
+__device__ inline float load_uniform_float(const float* table, uint32_t i) {
+    return __uint_as_float(((UniformWords)table)[i]);
+}
+
+// readfirstlane returns int: each half goes back through uint32_t before it
+// is widened, or a low word with bit 31 set would sign-extend over the high word.
+__device__ inline uint32_t uniform_u32(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
+__device__ inline __amdgpu_buffer_rsrc_t rf_resource(const void* base, uint32_t bytes) {
+    const uint64_t v = reinterpret_cast<uint64_t>(base);
+    const uint64_t u = (static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(v >> 32))) << 32) |
+                       static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(v)));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(u), 0, static_cast<int>(uniform_u32(bytes)),
+                                             kBufferDword3);
+}
+
+template <typename T> __device__ inline float load_rf(__amdgpu_buffer_rsrc_t r, uint32_t idx);
+template <> __device__ inline float load_rf<int16_t>(__amdgpu_buffer_rsrc_t r, uint32_t idx) {
+    return static_cast<float>(static_cast<int16_t>(
+        __builtin_amdgcn_raw_buffer_load_b16(r, static_cast<int>(idx * 2u), 0, 0)));
+}
+template <> __device__ inline float load_rf<float>(__amdgpu_buffer_rsrc_t r, uint32_t idx) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(idx * 4u), 0, 0));
+}
+
+// Linear interpolation of one channel at delay tau, with the reference's
+// range rules (RxSampleBeamformerDelayAndSum.h:174-187 for 2D, :147-161 for
+// 3D): i < N-1 interpolates, i == N-1 with zero fraction takes the sample,
+// anything else contributes nothing.  SAT_UINT selects the 3D conversion of
+// floor(tau) (saturating at 0, so negative delays read sample 0 with a
+// negative fraction) versus the 2D int32 conversion (negative delays are out
+// of range after the unsigned compare).
+template <typename RFT, bool SAT_UINT>
+__device__ inline void gather(__amdgpu_buffer_rsrc_t rf, uint32_t off, uint32_t n_s, float tau, float w,
+                              float& sample) {
+    const float fl = floorf(tau);
+    uint32_t ui;
+    float f;
+    if (SAT_UINT) {
+        ui = (fl > 0.0f) ? static_cast<uint32_t>(fl) : 0u;
+        f = tau - static_cast<float>(ui);
+    } else {
+        const int32_t i = static_cast<int32_t>(fl);
+        ui = static_cast<uint32_t>(i);
+        f = tau - static_cast<float>(i);
+    }
+    const float v0 = load_rf<RFT>(rf, off + ui);
+    const float v1 = load_rf<RFT>(rf, off + ui + 1u);
+    const bool inr = ui < n_s - 1u;
+    const bool edge = (ui == n_s - 1u) & (f == 0.0f);
+    const float val = edge ? v0 : ((1.0f - f) * v0 + f * v1);
+    const float contrib = w * val;
+    sample = (inr | edge) ? sample + contrib : sample;
 }
 
 struct Pixel2D {
     float d, aDT, inv, px, py2;
 };
 
-// One transmit's contribution for a 2D pixel, elements [e0, e1) of the
-// current LDS chunk (RxSampleBeamformerDelayAndSum.h:122-208).
-template <typename RFT>
-__device__ inline void accumulate2d(const RFT* __restrict__ rf, uint32_t n_s, float t0, float offset,
-                                    const Pixel2D& p, bool vertical, const float* s_xe, const float* s_dx,
-                                    const float* s_dx2, const uint32_t* s_off, const float* s_lut, int cnt,
-                                    float& sample, float& wsum, int& nadd) {
+// Elements [0, cnt) of the current LDS chunk for one 2D pixel
+// (RxSampleBeamformerDelayAndSum.h:150-199).
+template <typename RFT, bool FAR, bool VERT, bool MAP>
+__device__ inline void elements2d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float t0, float offset, const Pixel2D& p,
+                                  const float* s_xe, const float* s_dx, const float* s_dx2, const float* s_w,
+                                  const uint32_t* s_off, const float* s_lut, int cnt, float& sample, float& wsum,
+                                  int& nadd) {
+#pragma unroll 4
     for (int j = 0; j < cnt; j++) {
-        uint32_t off = s_off[j];
-        if (off == kNotConnected) continue;
-        float dx = s_dx[j];
-        if (fabsf(dx) <= p.aDT) {
-            float w = s_lut[window_index(dx * p.inv)];
-            wsum += w;
-            nadd++;
-            float a2;
-            if (vertical) {
-                a2 = s_dx2[j];
-            } else {
-                float a = s_xe[j] - p.px;
-                a2 = a * a;
-            }
-            float tau = t0 + (sqrt_rn(a2 + p.py2) + p.d);
-            tau = tau + offset;
-            float fl = floorf(tau);
-            int32_t i = static_cast<int32_t>(fl);
-            float f = tau - static_cast<float>(i);
-            if (static_cast<uint32_t>(i) < n_s - 1) {
-                float v0 = to_f(rf[off + static_cast<uint32_t>(i)]);
-                float v1 = to_f(rf[off + static_cast<uint32_t>(i) + 1]);
-                sample += w * ((1.0f - f) * v0 + f * v1);
-            } else if (static_cast<uint32_t>(i) < n_s && f == 0.0f) {
-                sample += w * to_f(rf[off + static_cast<uint32_t>(i)]);
-            }
+        const uint32_t off = s_off[j];
+        if (MAP && off == kNotConnected) continue;
+        float w;
+        if (FAR) {
+            w = s_w[j];
+        } else {
+            const float dx = s_dx[j];
+            if (!(fabsf(dx) <= p.aDT)) continue;
+            w = s_lut[window_index(dx * p.inv)];
         }
+        wsum += w;
+        nadd++;
+        float q;
+        if (VERT) {
+            q = s_dx2[j] + p.py2;
+        } else {
+            const float e = s_xe[j] - p.px;
+            q = e * e + p.py2;
+        }
+        float tau = t0 + (sqrt_rn(q) + p.d);
+        tau = tau + offset;
+        gather<RFT, false>(rf, off, n_s, tau, w, sample);
     }
 }
 
 // RxSampleBeamformerTestSignal::sampleBeamform2D (RxSampleBeamformerTestSignal.h:87-133):
 // 1000 inside 2 mm cylinders every 6 mm at 30 mm depth, 0 elsewhere.
-__device__ inline float test_signal_2d(float x, float dir_x, float dir_y, float dir_z, float d, float dt, float c) {
-    float px = ((x + dir_x * d) * dt) * c;
-    float py = ((dir_y * d) * dt) * c;
+__device__ inline float test_signal_2d(float x, float dir_x, float dir_y, float d, float dt, float c) {
+    const float px = ((x + dir_x * d) * dt) * c;
+    const float py = ((dir_y * d) * dt) * c;
     float sample = 0.0f;
     for (int k = -3; k <= 3; k++) {
-        float ex = px - static_cast<float>(k) * 6.0f;
-        float ey = py - 30.0f;
-        float ez = 0.0f;
-        float dist = sqrtf(ex * ex + ey * ey + ez * ez);
-        if (dist <= 2.0f) sample = 1000.0f;
+        const float ex = px - static_cast<float>(k) * 6.0f, ey = py - 30.0f, ez = 0.0f;
+        if (sqrtf(ex * ex + ey * ey + ez * ez) <= 2.0f) sample = 1000.0f;
     }
     return sample;
+}
+
+struct Tile {
+    uint32_t s, frame, r, r0;
+    bool valid;
+};
+
+__device__ inline Tile decode_tile(const DasArgs& a) {
+    Tile t;
+    const uint32_t bid = blockIdx.x;
+    const uint32_t tile = bid % a.depth_tiles;
+    const uint32_t rest = bid / a.depth_tiles;
+    t.s = rest % a.n_rx;
+    t.frame = rest / a.n_rx;
+    t.r0 = tile * kThreads;
+    t.r = t.r0 + threadIdx.x;
+    t.valid = t.r < a.n_depths;
+    return t;
 }
 
 template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
@@ -137,40 +259,41 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
     __shared__ float s_xe[kChunk2D];
     __shared__ float s_dx[kChunk2D];
     __shared__ float s_dx2[kChunk2D];
+    __shared__ float s_w[kChunk2D];
     __shared__ uint32_t s_off[kChunk2D];
 
-    const uint32_t bid = blockIdx.x;
-    const uint32_t tile = bid % a.depth_tiles;
-    const uint32_t rest = bid / a.depth_tiles;
-    const uint32_t s = rest % a.n_rx;
-    const uint32_t frame = rest / a.n_rx;
-    const uint32_t r = tile * kThreads + threadIdx.x;
-    const bool valid = r < a.n_depths;
-
+    const Tile t = decode_tile(a);
     for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = a.lut[i];
 
-    const ScanlineDT sc = a.scans[s];
+    const ScanView sc = scan_view(a.scans, t.s);
     Pixel2D p;
-    p.d = valid ? a.depths[r] : 0.0f;
+    p.d = t.valid ? a.depths[t.r] : 0.0f;
     p.aDT = (((p.d * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
-    p.inv = 1.0f / fminf(p.aDT, sc.max_ed_x);
-    p.px = sc.x + sc.dir_x * p.d;
-    float py = sc.dir_y * p.d;
+    p.inv = 1.0f / fminf(p.aDT, sc.max_ed_x());
+    p.px = sc.x() + sc.dir_x() * p.d;
+    const float py = sc.dir_y() * p.d;
     p.py2 = py * py;
-    const bool vertical = (sc.dir_x == 0.0f);
+    const bool vertical = (sc.dir_x() == 0.0f);
+    // aDT at the tile's first depth (the smallest one: the host enables the
+    // far-field path only for non-negative, non-decreasing depth tables)
+    const float d0 = load_uniform_float(a.depths, t.r0);
+    const float aDT0 = (((d0 * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    const float inv_far = 1.0f / sc.max_ed_x();
 
-    const RFT* rf_frame = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(frame) * a.rf_frame_stride;
+    const RFT* rf_frame = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(t.frame) * a.rf_frame_stride;
     float s_interp = 0.0f;
 
     for (int k = 0; k < 4; k++) {
-        const bool use = INTERP_TX ? ((sc.active_mask >> k) & 1) != 0 : (k == sc.sel);
+        const bool use = INTERP_TX ? ((sc.active_mask() >> k) & 1) != 0 : (k == sc.sel());
         if (!use) continue;   // uniform over the workgroup
-        const TxDT tx = sc.tx[k];
+        const TxDT tx = sc.tx(k);
         float s_local;
         if (TESTSIG) {
-            s_local = test_signal_2d(sc.x, sc.dir_x, sc.dir_y, sc.dir_z, p.d, a.dt, a.c);
+            s_local = test_signal_2d(sc.x(), sc.dir_x(), sc.dir_y(), p.d, a.dt, a.c);
         } else {
             const RFT* rf_tx = rf_frame + static_cast<uint64_t>(tx.tx_idx) * a.n_ch * a.n_s;
+            const __amdgpu_buffer_rsrc_t rsrc = rf_resource(rf_tx, a.n_ch * a.n_s * sizeof(RFT));
+            const bool far = a.far_ok && aDT0 >= tx.far_thr;
             float sample = 0.0f, wsum = 0.0f;
             int nadd = 0;
             const int first = tx.first_x, last = tx.last_x;
@@ -181,30 +304,44 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
                     const int e = base + j;
                     uint32_t ch;
                     if (MAP) {
-                        int32_t m = a.e2c[e];
+                        const int32_t m = a.e2c[e];
                         ch = (m < 0) ? kNotConnected : static_cast<uint32_t>(m);
                     } else {
                         ch = static_cast<uint32_t>(e) % a.n_ch;
                     }
                     const float xe = a.ex[e];
-                    const float dx = xe - sc.x;
+                    const float dx = xe - sc.x();
                     s_xe[j] = xe;
                     s_dx[j] = dx;
                     s_dx2[j] = dx * dx;
+                    s_w[j] = s_lut[window_index(dx * inv_far)];
                     s_off[j] = (ch == kNotConnected) ? kNotConnected : ch * a.n_s;
                 }
                 __syncthreads();
-                accumulate2d<RFT>(rf_tx, a.n_s, tx.t0, a.offset, p, vertical, s_xe, s_dx, s_dx2, s_off, s_lut,
-                                  cnt, sample, wsum, nadd);
+                if (far) {
+                    if (vertical)
+                        elements2d<RFT, true, true, MAP>(rsrc, a.n_s, tx.t0, a.offset, p, s_xe, s_dx, s_dx2, s_w,
+                                                         s_off, s_lut, cnt, sample, wsum, nadd);
+                    else
+                        elements2d<RFT, true, false, MAP>(rsrc, a.n_s, tx.t0, a.offset, p, s_xe, s_dx, s_dx2, s_w,
+                                                          s_off, s_lut, cnt, sample, wsum, nadd);
+                } else {
+                    if (vertical)
+                        elements2d<RFT, false, true, MAP>(rsrc, a.n_s, tx.t0, a.offset, p, s_xe, s_dx, s_dx2, s_w,
+                                                          s_off, s_lut, cnt, sample, wsum, nadd);
+                    else
+                        elements2d<RFT, false, false, MAP>(rsrc, a.n_s, tx.t0, a.offset, p, s_xe, s_dx, s_dx2, s_w,
+                                                           s_off, s_lut, cnt, sample, wsum, nadd);
+                }
             }
             s_local = (nadd > 0) ? (sample / wsum) * static_cast<float>(nadd) : 0.0f;
         }
-        if (INTERP_TX) s_interp += sc.txw[k] * s_local;
+        if (INTERP_TX) s_interp += sc.txw(k) * s_local;
         else s_interp += s_local;
     }
-    if (valid) {
-        OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(frame) * a.out_frame_stride;
-        out[static_cast<uint64_t>(r) * a.n_rx + s] = clamp_cast<OutT>(s_interp);
+    if (t.valid) {
+        OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(t.frame) * a.out_frame_stride;
+        out[static_cast<uint64_t>(t.r) * a.n_rx + t.s] = clamp_cast<OutT>(s_interp);
     }
 }
 
@@ -214,18 +351,47 @@ struct Pixel3D {
     float d, aDT, invx, invz, px, pz, py2;
 };
 
+// Elements [0, cnt) of the current LDS chunk for one 3D pixel
+// (RxSampleBeamformerDelayAndSum.h:55-111, x outer, y inner).
+template <typename RFT, bool FAR, bool MAP>
+__device__ inline void elements3d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float t0, float offset, const Pixel3D& p,
+                                  const float* s_xe, const float* s_ze, const float* s_dx, const float* s_dz,
+                                  const float* s_r2, const float* s_w, const uint32_t* s_off, const float* s_w2,
+                                  int cnt, float& sample, float& wsum, int& nadd) {
+#pragma unroll 4
+    for (int j = 0; j < cnt; j++) {
+        const uint32_t off = s_off[j];
+        if (MAP && off == kNotConnected) continue;
+        float w;
+        if (FAR) {
+            w = s_w[j];
+        } else {
+            if (!(s_r2[j] <= p.aDT)) continue;
+            const int ix = window_index(s_dx[j] * p.invx);
+            const int iz = window_index(s_dz[j] * p.invz);
+            w = s_w2[ix * kW2Stride + iz];
+        }
+        wsum += w;
+        nadd++;
+        const float ax = s_xe[j] - p.px, az = s_ze[j] - p.pz;
+        float tau = t0 + (sqrt_rn((ax * ax + az * az) + p.py2) + p.d);
+        tau = tau + offset;
+        gather<RFT, true>(rf, off, n_s, tau, w, sample);
+    }
+}
+
 // RxSampleBeamformerTestSignal::sampleBeamform3D (RxSampleBeamformerTestSignal.h:25-85).
-__device__ inline float test_signal_3d(const ScanlineDT& sc, float d, float dt, float c) {
-    float px = ((sc.x + sc.dir_x * d) * dt) * c;
-    float py = ((sc.dir_y * d) * dt) * c;
-    float pz = ((sc.z + sc.dir_z * d) * dt) * c;
+__device__ inline float test_signal_3d(const ScanView& sc, float d, float dt, float c) {
+    const float px = ((sc.x() + sc.dir_x() * d) * dt) * c;
+    const float py = ((sc.dir_y() * d) * dt) * c;
+    const float pz = ((sc.z() + sc.dir_z() * d) * dt) * c;
     float sample = 0.0f;
     for (int k = -3; k <= 3; k++) {
-        float ex = px - static_cast<float>(k) * 6.0f, ey = py - 30.0f, ez = 0.0f;
+        const float ex = px - static_cast<float>(k) * 6.0f, ey = py - 30.0f, ez = 0.0f;
         if (sqrtf(ex * ex + ey * ey + ez * ez) <= 2.0f) sample = 1000.0f;
     }
     for (int k = -3; k <= 3; k++) {
-        float ex = 0.0f, ey = py - 30.0f, ez = pz - static_cast<float>(k) * 6.0f;
+        const float ex = 0.0f, ey = py - 30.0f, ez = pz - static_cast<float>(k) * 6.0f;
         if (sqrtf(ex * ex + ey * ey + ez * ez) <= 2.0f) sample = 1000.0f;
     }
     return sample;
@@ -233,61 +399,65 @@ __device__ inline float test_signal_3d(const ScanlineDT& sc, float d, float dt, 
 
 template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
 __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
-    __shared__ float s_w2[kWindowEntries * kWindowEntries];   // sqrt(LUT[i] * LUT[j])
+    __shared__ float s_w2[kWindowEntries * kW2Stride];   // sqrt(LUT[i] * LUT[j]), padded rows
     __shared__ float s_xe[kChunk3D];
     __shared__ float s_ze[kChunk3D];
+    __shared__ float s_dx[kChunk3D];
+    __shared__ float s_dz[kChunk3D];
     __shared__ float s_r2[kChunk3D];
+    __shared__ float s_w[kChunk3D];
     __shared__ uint32_t s_off[kChunk3D];
 
-    const uint32_t bid = blockIdx.x;
-    const uint32_t tile = bid % a.depth_tiles;
-    const uint32_t rest = bid / a.depth_tiles;
-    const uint32_t s = rest % a.n_rx;
-    const uint32_t frame = rest / a.n_rx;
-    const uint32_t r = tile * kThreads + threadIdx.x;
-    const bool valid = r < a.n_depths;
-
+    const Tile t = decode_tile(a);
     // computeWindow3DShared (RxBeamformerCommon.h:54-61) as a 64x64 table:
     // the product and the square root are the same IEEE operations.
-    for (int i = threadIdx.x; i < kWindowEntries * kWindowEntries; i += kThreads)
-        s_w2[i] = sqrtf(a.lut[i / kWindowEntries] * a.lut[i % kWindowEntries]);
-
-    const ScanlineDT sc = a.scans[s];
-    Pixel3D p;
-    p.d = valid ? a.depths[r] : 0.0f;
-    {
-        float t = (((p.d * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
-        p.aDT = t * t;
-        float sq = sqrtf(p.aDT);
-        p.invx = 1.0f / fminf(sq, sc.max_ed_x);
-        p.invz = 1.0f / fminf(sq, sc.max_ed_y);
+    for (int i = threadIdx.x; i < kWindowEntries * kWindowEntries; i += kThreads) {
+        const int ix = i / kWindowEntries, iz = i % kWindowEntries;
+        s_w2[ix * kW2Stride + iz] = sqrtf(a.lut[ix] * a.lut[iz]);
     }
-    p.px = sc.x + sc.dir_x * p.d;
-    p.pz = sc.z + sc.dir_z * p.d;
+
+    const ScanView sc = scan_view(a.scans, t.s);
+    Pixel3D p;
+    p.d = t.valid ? a.depths[t.r] : 0.0f;
     {
-        float py = sc.dir_y * p.d;
+        const float ap = (((p.d * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+        p.aDT = ap * ap;
+        const float sq = sqrtf(p.aDT);
+        p.invx = 1.0f / fminf(sq, sc.max_ed_x());
+        p.invz = 1.0f / fminf(sq, sc.max_ed_y());
+    }
+    p.px = sc.x() + sc.dir_x() * p.d;
+    p.pz = sc.z() + sc.dir_z() * p.d;
+    {
+        const float py = sc.dir_y() * p.d;
         p.py2 = py * py;
     }
+    const float d0 = load_uniform_float(a.depths, t.r0);
+    const float ap0 = (((d0 * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    const float aDT0 = ap0 * ap0;
+    const float sq0 = sqrtf(aDT0);
+    const float invx_far = 1.0f / sc.max_ed_x(), invz_far = 1.0f / sc.max_ed_y();
 
-    const RFT* rf_frame = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(frame) * a.rf_frame_stride;
+    const RFT* rf_frame = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(t.frame) * a.rf_frame_stride;
     float s_interp = 0.0f;
 
     for (int k = 0; k < 4; k++) {
-        const bool use = INTERP_TX ? ((sc.active_mask >> k) & 1) != 0 : (k == sc.sel);
+        const bool use = INTERP_TX ? ((sc.active_mask() >> k) & 1) != 0 : (k == sc.sel());
         if (!use) continue;
-        const TxDT tx = sc.tx[k];
+        const TxDT tx = sc.tx(k);
         float s_local;
         if (TESTSIG) {
             s_local = test_signal_3d(sc, p.d, a.dt, a.c);
         } else {
             const RFT* rf_tx = rf_frame + static_cast<uint64_t>(tx.tx_idx) * a.n_ch * a.n_s;
+            const __amdgpu_buffer_rsrc_t rsrc = rf_resource(rf_tx, a.n_ch * a.n_s * sizeof(RFT));
+            const bool far = a.far_ok && aDT0 >= tx.far_thr && sq0 >= sc.max_ed_x() && sq0 >= sc.max_ed_y();
             float sample = 0.0f, wsum = 0.0f;
             int nadd = 0;
             const int ny = (tx.last_y > tx.first_y) ? tx.last_y - tx.first_y : 0;
             const int nx = (tx.last_x > tx.first_x) ? tx.last_x - tx.first_x : 0;
             const int total = nx * ny;
             // element j of the aperture: ex = first_x + j / ny, ey = first_y + j % ny
-            // (x outer, y inner: RxSampleBeamformerDelayAndSum.h:55-59)
             for (int base = 0; base < total; base += kChunk3D) {
                 const int cnt = min(kChunk3D, total - base);
                 __syncthreads();
@@ -298,57 +468,37 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
                     const uint32_t e = exi + eyi * a.layout_x;
                     uint32_t ch;
                     if (MAP) {
-                        int32_t m = a.e2c[e];
+                        const int32_t m = a.e2c[e];
                         ch = (m < 0) ? kNotConnected : static_cast<uint32_t>(m);
                     } else {
                         ch = e % a.n_ch;
                     }
                     const float xe = a.ex[e], ze = a.ey[e];
-                    const float dx = xe - sc.x, dz = ze - sc.z;
+                    const float dx = xe - sc.x(), dz = ze - sc.z();
                     s_xe[j] = xe;
                     s_ze[j] = ze;
+                    s_dx[j] = dx;
+                    s_dz[j] = dz;
                     s_r2[j] = dx * dx + dz * dz;
+                    s_w[j] = s_w2[window_index(dx * invx_far) * kW2Stride + window_index(dz * invz_far)];
                     s_off[j] = (ch == kNotConnected) ? kNotConnected : ch * a.n_s;
                 }
                 __syncthreads();
-                for (int j = 0; j < cnt; j++) {
-                    const uint32_t off = s_off[j];
-                    if (off == kNotConnected) continue;
-                    if (s_r2[j] <= p.aDT) {
-                        const float xe = s_xe[j], ze = s_ze[j];
-                        const int ix = window_index((xe - sc.x) * p.invx);
-                        const int iz = window_index((ze - sc.z) * p.invz);
-                        const float w = s_w2[ix * kWindowEntries + iz];
-                        wsum += w;
-                        nadd++;
-                        const float ax = xe - p.px, az = ze - p.pz;
-                        float tau = tx.t0 + (sqrt_rn((ax * ax + az * az) + p.py2) + p.d);
-                        tau = tau + a.offset;
-                        // floor then a saturating float->uint32 conversion, as the
-                        // reference's static_cast<uint32_t>(floor(.)) on the GPU
-                        // (RxSampleBeamformerDelayAndSum.h:87-88): a negative delay
-                        // becomes sample 0 with a negative fraction.
-                        const float fl = floorf(tau);
-                        const uint32_t i = (fl > 0.0f) ? static_cast<uint32_t>(fl) : 0u;
-                        const float f = tau - static_cast<float>(i);
-                        if (i < a.n_s - 1) {
-                            const float v0 = to_f(rf_tx[off + i]);
-                            const float v1 = to_f(rf_tx[off + i + 1]);
-                            sample += w * ((1.0f - f) * v0 + f * v1);
-                        } else if (i < a.n_s && f == 0.0f) {
-                            sample += w * to_f(rf_tx[off + i]);
-                        }
-                    }
-                }
+                if (far)
+                    elements3d<RFT, true, MAP>(rsrc, a.n_s, tx.t0, a.offset, p, s_xe, s_ze, s_dx, s_dz, s_r2, s_w,
+                                               s_off, s_w2, cnt, sample, wsum, nadd);
+                else
+                    elements3d<RFT, false, MAP>(rsrc, a.n_s, tx.t0, a.offset, p, s_xe, s_ze, s_dx, s_dz, s_r2, s_w,
+                                                s_off, s_w2, cnt, sample, wsum, nadd);
             }
             s_local = (nadd > 0) ? (sample / wsum) * static_cast<float>(nadd) : 0.0f;
         }
-        if (INTERP_TX) s_interp += sc.txw[k] * s_local;
+        if (INTERP_TX) s_interp += sc.txw(k) * s_local;
         else s_interp += s_local;
     }
-    if (valid) {
-        OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(frame) * a.out_frame_stride;
-        out[static_cast<uint64_t>(r) * a.n_rx + s] = clamp_cast<OutT>(s_interp);
+    if (t.valid) {
+        OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(t.frame) * a.out_frame_stride;
+        out[static_cast<uint64_t>(t.r) * a.n_rx + t.s] = clamp_cast<OutT>(s_interp);
     }
 }
 
@@ -359,18 +509,18 @@ static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool i
                                hipStream_t stream) {
     const uint64_t blocks = static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames;
     dim3 grid(static_cast<uint32_t>(blocks)), block(kThreads);
-#define SUPRA_DAS_DISPATCH(KERNEL)                                                              \
-    do {                                                                                         \
-        if (testsig) {                                                                           \
+#define SUPRA_DAS_DISPATCH(KERNEL)                                                                                 \
+    do {                                                                                                            \
+        if (testsig) {                                                                                              \
             if (interp_tx) hipLaunchKernelGGL((KERNEL<RFT, OutT, true, true, false>), grid, block, 0, stream, a);   \
             else hipLaunchKernelGGL((KERNEL<RFT, OutT, true, false, false>), grid, block, 0, stream, a);            \
-        } else if (map) {                                                                        \
+        } else if (map) {                                                                                           \
             if (interp_tx) hipLaunchKernelGGL((KERNEL<RFT, OutT, false, true, true>), grid, block, 0, stream, a);   \
             else hipLaunchKernelGGL((KERNEL<RFT, OutT, false, false, true>), grid, block, 0, stream, a);            \
-        } else {                                                                                 \
+        } else {                                                                                                    \
             if (interp_tx) hipLaunchKernelGGL((KERNEL<RFT, OutT, false, true, false>), grid, block, 0, stream, a);  \
             else hipLaunchKernelGGL((KERNEL<RFT, OutT, false, false, false>), grid, block, 0, stream, a);           \
-        }                                                                                        \
+        }                                                                                                           \
     } while (0)
     if (is3d) SUPRA_DAS_DISPATCH(das3d_kernel);
     else SUPRA_DAS_DISPATCH(das2d_kernel);
