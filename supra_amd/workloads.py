@@ -160,7 +160,8 @@ def matrix_3d(**overrides) -> Workload:
     return replace(w, **overrides) if overrides else w
 
 
-def matrix_3d_small(n_el: int = 8, n_lines: int = 8, n_depths: int = 200, **overrides) -> Workload:
+def matrix_3d_small(n_el: int = 8, n_lines: int = 8, n_depths: int = 200, n_samples: int = 1100,
+                    **overrides) -> Workload:
     """Reduced matrix probe with the same physics (host-oracle sized)."""
     sector = 2.0 * math.degrees(math.atan(20.0 / 40.0))
     bf = BeamformerConfig(scan_type="biphased", transducer_type="planar", element_layout=(n_el, n_el),
@@ -169,7 +170,7 @@ def matrix_3d_small(n_el: int = 8, n_lines: int = 8, n_depths: int = 200, **over
                           sector_angle_deg=(sector, sector), tx_focus_active=True, tx_focus_depth=20.0,
                           speed_of_sound=1540.0)
     w = Workload(name=f"matrix_3d_{n_el}x{n_el}_{n_lines}x{n_lines}x{n_depths}", bf=bf, fs_hz=20e6,
-                 num_channels=n_el * n_el, num_samples=n_depths, pulse_hz=3.5e6, num_scatterers=60,
+                 num_channels=n_el * n_el, num_samples=n_samples, pulse_hz=3.5e6, num_scatterers=60,
                  box_x=(-20.0, 20.0), box_depth=(1.0, 40.0), box_z=(-20.0, 20.0), scatter_seed=2,
                  noise_seed=3, beam_sigma_mm=1.5)
     return replace(w, **overrides) if overrides else w
