@@ -181,6 +181,34 @@ int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer, uint32_t 
                          int32_t interpolate_transmits, int32_t additional_offset,
                          void* out, int32_t out_dtype, void* stream);
 
+/* RawDelay::performDelay<ChannelDataType, OutputType>
+ * (src/SupraLib/Beamformer/RawDelay.cu:555-650; kernel :404-468, per-sample
+ * math :221-312).  2D geometries.  For each rx scanline's highest-weight
+ * transmit, writes every aperture element's delayed, apodised sample times
+ * n/W: out[scanline][j][depth] (device, out_dtype), j = element - first;
+ * channels j >= last - first are written as 0.  out_sampling_frequency_hz
+ * (may be NULL) receives the reference's output rate 1/(depth/c/num_depths)
+ * (RawDelay.cu:642). */
+int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype,
+                       uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                       double sampling_frequency_hz, double f_number, double speed_of_sound_mm_per_s,
+                       int32_t window_type, float window_parameter,
+                       void* out, int32_t out_dtype, double* out_sampling_frequency_hz, void* stream);
+
+/* RxBeamformerMV::performRxBeamforming<ChannelDataType, ImageDataType>
+ * (src/SupraLib/Beamformer/RxBeamformerMV.cu:382-614, kernels :79-378, the
+ * cuBLAS solve :558-586), with the cublasHandle_t argument dropped and a
+ * diagonal-loading factor added (1.0 = the reference: trace(R)/L_t).
+ * raw_delayed: device [num_scanlines][num_channels][num_samples] (the output
+ * of supra_rawdelay_run); out: device [num_samples][num_scanlines].
+ * sub_array_size 0 selects num_channels/2 (BeamformingMVNode default).
+ * One fused launch; no host synchronisation. */
+int supra_mv_run(const void* raw_delayed, int32_t in_dtype,
+                 uint32_t num_scanlines, uint32_t num_channels, uint32_t num_samples,
+                 uint32_t sub_array_size, uint32_t temporal_smoothing,
+                 double sub_array_scaling_power, int32_t compute_means, double loading_factor,
+                 void* out, int32_t out_dtype, void* stream);
+
 /* Window LUT of the reference (WindowFunction.cpp:18-47, WindowFunction.h:122-138),
  * computed on the host: SUPRA_WINDOW_NUM_ENTRIES floats, max-normalised. */
 int supra_window_lut(int32_t window_type, float window_parameter, float* lut_out);

@@ -382,6 +382,217 @@ void das_run(const DasCall& call, const uint32_t* pixels, uint64_t n_pixels, T* 
     });
 }
 
+// One pixel of the 2D pre-delay: rxDelayDTSPACEKernel + sampleDelay2D
+// (src/SupraLib/Beamformer/RawDelay.cu:221-312, 404-468).  Highest-weight
+// transmit only, no additional offset; writes w*interp*(n/W) for aperture
+// element j = e - first into out[(s*N_ch + j)*N_d + r]; slots j >= last-first
+// (never written by the reference, RawDelay.cu:570-572) are defined as 0.
+// Pinned by: tests/test_oracle_mv.py::test_rawdelay_sums_to_das — summing the
+// channels of the pre-delayed data gives the DAS image (same F-number, window,
+// no offset), and a constant RF gives w_e * k * n / W per channel.
+template <typename T>
+void rawdelay_pixel(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32_t s, uint32_t r, T* out) {
+    const supra_rx_geometry& g = *call.g;
+    const supra_scanline_rx& sc = g.scanlines[s];
+    const uint32_t n_ch = call.rf.n_ch, n_d = g.num_depths;
+    T* dst = out + static_cast<uint64_t>(s) * n_ch * n_d + r;
+    for (uint32_t j = 0; j < n_ch; j++) dst[static_cast<uint64_t>(j) * n_d] = 0;
+    const T dt = static_cast<T>(1.0 / call.fs), c = static_cast<T>(call.c), F = static_cast<T>(call.F);
+    const T d = G.depth[r];
+    const T aDT = (((d * dt) * c) / (T(2) * F)) / c / dt;
+    const T x = G.sx[s], dirx = G.dirx[s], diry = G.diry[s];
+    const T inv = T(1) / vmin(aDT, G.medx[s]);
+    const int k = select_tx(sc);
+    if (!(sc.tx_weights[k] > 0.0)) return;
+    const supra_tx_params& tp = sc.tx[k];
+    const T t0 = G.t0[4 * s + k];
+    T wsum = 0;
+    int nadd = 0;
+    for (int32_t e = tp.first_active_element[0]; e < tp.last_active_element[0]; e++) {
+        const T xe = G.ex[e];
+        if (std::fabs(xe - x) <= aDT) {
+            wsum += window_get(lut, (xe - x) * inv);
+            nadd++;
+        }
+    }
+    T scale = T(1);
+    if (nadd > 0) scale = T(1) / wsum * static_cast<T>(nadd);
+    const uint64_t tx_base = static_cast<uint64_t>(tp.tx_scanline_idx) * n_ch * call.rf.n_s;
+    uint32_t j = 0;
+    for (int32_t e = tp.first_active_element[0]; e < tp.last_active_element[0]; e++, j++) {
+        T sample = 0;
+        const uint32_t ch = static_cast<uint32_t>(e) % n_ch;
+        const T xe = G.ex[e];
+        if (std::fabs(xe - x) <= aDT) {
+            const T w = window_get(lut, (xe - x) * inv);
+            const T a = xe - (x + dirx * d);
+            const T b = diry * d;
+            const T tau = t0 + (vsqrt(a * a + b * b) + d);
+            const int32_t i = static_cast<int32_t>(vfloor(tau));
+            const T f = tau - static_cast<T>(i);
+            const uint64_t base = tx_base + static_cast<uint64_t>(ch) * call.rf.n_s;
+            if (static_cast<uint32_t>(i) < call.rf.n_s - 1) {
+                sample = w * ((T(1) - f) * rf_at<T>(call.rf, base + static_cast<uint32_t>(i)) +
+                              f * rf_at<T>(call.rf, base + static_cast<uint32_t>(i) + 1));
+            } else if (static_cast<uint32_t>(i) < call.rf.n_s && f == T(0)) {
+                sample = w * rf_at<T>(call.rf, base + static_cast<uint32_t>(i));
+            }
+        }
+        dst[static_cast<uint64_t>(j) * n_d] = clamp_out(sample * scale, call.out_dtype);
+    }
+}
+
+template <typename T>
+void rawdelay_run(const DasCall& call, T* out, int nthreads) {
+    const supra_rx_geometry& g = *call.g;
+    DtGeometry<T> G = to_dt_space<T>(g, 1.0 / call.fs, call.c);
+    T lut[kEntries];
+    window_lut<T>(call.window, call.wparam, lut);
+    const uint64_t total = static_cast<uint64_t>(g.num_rx_scanlines) * g.num_depths;
+    parallel_for(static_cast<int64_t>(total), nthreads, [&](int64_t q) {
+        const uint32_t s = static_cast<uint32_t>(q / g.num_depths), r = static_cast<uint32_t>(q % g.num_depths);
+        rawdelay_pixel<T>(call, G, lut, s, r, out);
+    });
+}
+
+// Minimum variance over pre-delayed data x[scanline][channel][sample], FP64,
+// with the reference's intended per-sample math over the full depth
+// (src/SupraLib/Beamformer/RxBeamformerMV.cu:44-378):
+//   aperture = first/last channel with x != 0 (:44-77); L_t = min(L, last-first+1);
+//   subarray k active iff first+k+L_t-1 <= last (the offsets of active subarrays
+//   are first+k, so the reference's clamp and de-duplication (:98-110) never
+//   change them); R(t) = sum_k (x_k - m)(x_k - m)^T over active subarrays
+//   (:151-195, means optional :115-149); R~(t) = sum of R(tau) for
+//   |tau - t| <= T inside the frame, each R(tau) zero outside its own
+//   L_tau x L_tau block, restricted to L_t x L_t (:197-228); loading
+//   delta = alpha * trace(R~)/L_t on the diagonal (:239-278; alpha = 1 in the
+//   reference); w = R~^-1 1 by Gaussian elimination with partial pivoting
+//   (the cuBLAS getrf/getrs of :558-586); y = sum_i (sum_k x[first+k+i]) w_i /
+//   ((sum_i w_i) K_act^p); y := 0 if |y| > 1e7 or NaN; out = clampCast(y*N_ch)
+//   (:307-378).
+// Not reproduced (reference batch bugs for N_s > 2000, :87,146,165,171,173,
+// 184-185,361): sample indices and the temporal window are per frame here.
+// Pinned by: tests/test_oracle_mv.py — plane-wave closed form
+// out = N_ch s(t) K^(1-p) (independent of alpha and T), the alpha -> infinity
+// closed form y = sum_i X_i / (L_t K^p), numpy.linalg.solve on the assembled
+// systems, and a numpy restatement of R~ on random data.
+struct MvCall {
+    const void* x;
+    int dtype;
+    uint32_t n_rx, n_ch, n_d, L, T;
+    double p, alpha;
+    int means, out_dtype;
+};
+
+inline double x_at(const MvCall& m, uint64_t idx) {
+    return m.dtype == SUPRA_INT16 ? static_cast<const int16_t*>(m.x)[idx] : static_cast<const float*>(m.x)[idx];
+}
+
+struct MvSample {
+    uint32_t first, last, Lt, Kact;
+};
+
+// Solve A w = b (n x n, row-major, A and b overwritten) with partial pivoting.
+void solve_lu(double* A, double* b, int n) {
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+        if (piv != c) {
+            for (int k = 0; k < n; k++) std::swap(A[c * n + k], A[piv * n + k]);
+            std::swap(b[c], b[piv]);
+        }
+        const double d = A[c * n + c];
+        for (int r = c + 1; r < n; r++) {
+            const double f = A[r * n + c] / d;
+            for (int k = c; k < n; k++) A[r * n + k] -= f * A[c * n + k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int r = n - 1; r >= 0; r--) {
+        double acc = b[r];
+        for (int k = r + 1; k < n; k++) acc -= A[r * n + k] * b[k];
+        b[r] = acc / A[r * n + r];
+    }
+}
+
+void mv_scanline(const MvCall& m, uint32_t s, double* out, uint32_t dbg_sample, double* dbg_A, double* dbg_w,
+                 int32_t* dbg_meta) {
+    const uint32_t n_ch = m.n_ch, n_d = m.n_d, L = m.L;
+    const uint32_t K = n_ch - L + 1;
+    const uint64_t base = static_cast<uint64_t>(s) * n_ch * n_d;
+    auto X = [&](uint32_t ch, uint32_t t) { return x_at(m, base + static_cast<uint64_t>(ch) * n_d + t); };
+    std::vector<MvSample> meta(n_d);
+    std::vector<double> R(static_cast<size_t>(n_d) * L * L, 0.0);
+    std::vector<double> mean(L);
+    for (uint32_t t = 0; t < n_d; t++) {
+        uint32_t f = 0, l = 0;
+        for (uint32_t c = 0; c < n_ch; c++) if (X(c, t) != 0) { f = c; break; }
+        for (int32_t c = static_cast<int32_t>(n_ch) - 1; c >= 0; c--) if (X(c, t) != 0) { l = c; break; }
+        MvSample ms;
+        ms.first = f;
+        ms.last = l;
+        ms.Lt = std::min(L, l - f + 1);
+        ms.Kact = 0;
+        for (uint32_t k = 0; k < K; k++) if (f + k + ms.Lt - 1 <= l) ms.Kact++;
+        meta[t] = ms;
+        std::fill(mean.begin(), mean.end(), 0.0);
+        if (m.means) {
+            for (uint32_t i = 0; i < ms.Lt; i++) {
+                double acc = 0;
+                for (uint32_t k = 0; k < ms.Kact; k++) acc += X(f + k + i, t);
+                mean[i] = acc / ms.Kact;
+            }
+        }
+        double* Rt = &R[static_cast<size_t>(t) * L * L];
+        for (uint32_t k = 0; k < ms.Kact; k++)
+            for (uint32_t i = 0; i < ms.Lt; i++) {
+                const double xi = X(f + k + i, t) - mean[i];
+                for (uint32_t j = 0; j < ms.Lt; j++) Rt[i * L + j] += xi * (X(f + k + j, t) - mean[j]);
+            }
+    }
+    std::vector<double> A(L * L), w(L);
+    for (uint32_t t = 0; t < n_d; t++) {
+        const MvSample ms = meta[t];
+        const uint32_t n = ms.Lt;
+        const uint32_t t_lo = t >= m.T ? t - m.T : 0;
+        const uint32_t t_hi = std::min(n_d - 1, t + m.T);
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t j = 0; j < n; j++) {
+                double acc = 0;
+                for (uint32_t tau = t_lo; tau <= t_hi; tau++) acc += R[static_cast<size_t>(tau) * L * L + i * L + j];
+                A[i * n + j] = acc;
+            }
+        double trace = 0;
+        for (uint32_t i = 0; i < n; i++) trace += A[i * n + i];
+        const double delta = m.alpha * (trace / n);
+        for (uint32_t i = 0; i < n; i++) A[i * n + i] += delta;
+        if (t == dbg_sample && dbg_A) std::copy(A.begin(), A.begin() + n * n, dbg_A);
+        for (uint32_t i = 0; i < n; i++) w[i] = 1.0;
+        solve_lu(A.data(), w.data(), static_cast<int>(n));
+        double wsum = 0;
+        for (uint32_t i = 0; i < n; i++) wsum += w[i];
+        const double scaling = 1.0 / (wsum * std::pow(static_cast<double>(ms.Kact), m.p));
+        double y = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            double xs = 0;
+            for (uint32_t k = 0; k < ms.Kact; k++) xs += X(ms.first + k + i, t);
+            y += xs * w[i] * scaling;
+        }
+        if (std::fabs(y) > 1e7 || std::isnan(y)) y = 0;
+        if (t == dbg_sample) {
+            if (dbg_w) std::copy(w.begin(), w.begin() + n, dbg_w);
+            if (dbg_meta) {
+                dbg_meta[0] = static_cast<int32_t>(ms.first);
+                dbg_meta[1] = static_cast<int32_t>(ms.last);
+                dbg_meta[2] = static_cast<int32_t>(ms.Lt);
+                dbg_meta[3] = static_cast<int32_t>(ms.Kact);
+            }
+        }
+        out[static_cast<uint64_t>(t) * m.n_rx + s] = clamp_out(y * n_ch, m.out_dtype);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -408,6 +619,48 @@ int oracle_das(const supra_rx_geometry* g, const void* rf, int rf_dtype, uint32_
                  interpolate_transmits, additional_offset, test_signal, out_dtype};
     if (precision == 64) das_run<double>(call, pixels, n_pixels, static_cast<double*>(out), nthreads);
     else das_run<float>(call, pixels, n_pixels, static_cast<float*>(out), nthreads);
+    return 0;
+}
+
+
+// Pre-delayed RF out[scanline][channel][depth] (2D geometries), float
+// (precision 32) or double (precision 64).
+int oracle_rawdelay(const supra_rx_geometry* g, const void* rf, int rf_dtype, uint32_t n_tx, uint32_t n_ch,
+                    uint32_t n_s, double fs, double f_number, double c_mm_per_s, int window_type,
+                    float window_parameter, int out_dtype, int precision, void* out, int nthreads) {
+    if (!g || !rf || !out || n_tx == 0 || n_ch == 0 || n_s == 0) return -1;
+    if (g->rx_scanline_layout[0] > 1 && g->rx_scanline_layout[1] > 1) return -3;
+    for (uint32_t i = 0; i < g->num_rx_scanlines; i++) {
+        const int k = select_tx(g->scanlines[i]);
+        const supra_tx_params& t = g->scanlines[i].tx[k];
+        if (g->scanlines[i].tx_weights[k] > 0.0 && t.tx_scanline_idx >= n_tx) return -2;
+        if (t.last_active_element[0] > t.first_active_element[0] &&
+            static_cast<uint32_t>(t.last_active_element[0] - t.first_active_element[0]) > n_ch) return -2;
+    }
+    DasCall call{g, RfView{rf, rf_dtype, n_tx, n_ch, n_s}, fs, f_number, c_mm_per_s, window_type, window_parameter,
+                 0, 0, 0, out_dtype};
+    if (precision == 64) rawdelay_run<double>(call, static_cast<double*>(out), nthreads);
+    else rawdelay_run<float>(call, static_cast<float*>(out), nthreads);
+    return 0;
+}
+
+// Minimum variance image out[depth][scanline] (double) from pre-delayed data
+// x[scanline][channel][depth].  debug_*: the system (n x n), weights and
+// (first, last, L_t, K_act) of sample `debug_sample` of scanline `debug_scanline`.
+int oracle_mv(const void* x, int dtype, uint32_t n_rx, uint32_t n_ch, uint32_t n_d, uint32_t sub_array_size,
+              uint32_t temporal_smoothing, double scaling_power, int compute_means, double loading_factor,
+              int out_dtype, double* out, int nthreads, uint32_t debug_scanline, uint32_t debug_sample,
+              double* debug_A, double* debug_w, int32_t* debug_meta) {
+    if (!x || !out || n_rx == 0 || n_ch == 0 || n_d == 0) return -1;
+    uint32_t L = sub_array_size == 0 ? n_ch / 2 : sub_array_size;
+    if (L == 0 || L > n_ch) return -1;
+    MvCall m{x, dtype, n_rx, n_ch, n_d, L, temporal_smoothing, scaling_power, loading_factor, compute_means,
+             out_dtype};
+    parallel_for(static_cast<int64_t>(n_rx), nthreads, [&](int64_t s) {
+        const bool dbg = static_cast<uint32_t>(s) == debug_scanline;
+        mv_scanline(m, static_cast<uint32_t>(s), out, dbg ? debug_sample : 0xffffffffu, dbg ? debug_A : nullptr,
+                    dbg ? debug_w : nullptr, dbg ? debug_meta : nullptr);
+    });
     return 0;
 }
 

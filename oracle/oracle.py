@@ -35,6 +35,14 @@ def lib() -> C.CDLL:
         h.oracle_das.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.c_double, C.c_double, C.c_double, C.c_int, C.c_float, C.c_int, C.c_int32,
                                  C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]
+        h.oracle_rawdelay.restype = C.c_int
+        h.oracle_rawdelay.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_double, C.c_double, C.c_double, C.c_int, C.c_float, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_int]
+        h.oracle_mv.restype = C.c_int
+        h.oracle_mv.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.c_double, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int, C.c_uint32,
+                                C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = h
     return _lib
 
@@ -73,4 +81,46 @@ def das(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_p
                             out.ctypes.data, threads or default_threads())
     if code != 0:
         raise ValueError(f"oracle_das rejected its arguments ({code})")
+    return out
+
+
+def rawdelay(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_per_s: float,
+             window_type: int, window_parameter: float = 0.0, out_int16: bool = False, precision: int = 32,
+             threads: int | None = None) -> np.ndarray:
+    """Oracle pre-delay [num_rx, num_channels, num_depths] (2D geometries)."""
+    rf = np.ascontiguousarray(rf)
+    gc = geom.to_c()
+    out = np.zeros((geom.num_rx, rf.shape[1], geom.num_depths), dtype=np.float32 if precision == 32 else np.float64)
+    code = lib().oracle_rawdelay(C.addressof(gc), rf.ctypes.data, 1 if rf.dtype == np.int16 else 2, rf.shape[0],
+                                 rf.shape[1], rf.shape[2], fs_hz, f_number, speed_of_sound_mm_per_s, window_type,
+                                 window_parameter, 1 if out_int16 else 2, precision, out.ctypes.data,
+                                 threads or default_threads())
+    if code != 0:
+        raise ValueError(f"oracle_rawdelay rejected its arguments ({code})")
+    return out
+
+
+def mv(x: np.ndarray, sub_array_size: int = 0, temporal_smoothing: int = 3, scaling_power: float = 1.5,
+       compute_means: bool = False, loading_factor: float = 1.0, out_int16: bool = False,
+       threads: int | None = None, debug: tuple | None = None):
+    """Oracle minimum variance (FP64) on x [scanline, channel, sample] -> [sample, scanline].
+    debug=(scanline, sample) also returns (A, w, (first, last, L_t, K_act)) of that sample."""
+    x = np.ascontiguousarray(x)
+    assert x.ndim == 3 and x.dtype in (np.int16, np.float32)
+    n_rx, n_ch, n_d = x.shape
+    L = sub_array_size or n_ch // 2
+    out = np.zeros((n_d, n_rx))
+    dA = np.zeros(L * L)
+    dw = np.zeros(L)
+    dm = np.zeros(4, dtype=np.int32)
+    ds, dt = debug if debug else (0xFFFFFFFF, 0xFFFFFFFF)
+    code = lib().oracle_mv(x.ctypes.data, 1 if x.dtype == np.int16 else 2, n_rx, n_ch, n_d, sub_array_size,
+                           temporal_smoothing, scaling_power, int(compute_means), loading_factor,
+                           1 if out_int16 else 2, out.ctypes.data, threads or default_threads(), ds, dt,
+                           dA.ctypes.data, dw.ctypes.data, dm.ctypes.data)
+    if code != 0:
+        raise ValueError(f"oracle_mv rejected its arguments ({code})")
+    if debug:
+        n = int(dm[2])
+        return out, dA[:n * n].reshape(n, n), dw[:n], tuple(int(v) for v in dm)
     return out

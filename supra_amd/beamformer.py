@@ -222,6 +222,105 @@ class BeamformingNode:
         return img
 
 
+class RawDelay:
+    """RawDelay (src/SupraLib/Beamformer/RawDelay.h:38-48): per-channel delayed,
+    apodised RF [scanline][channel][depth] for the MV beamformer."""
+
+    def __init__(self, geometry: RxGeometry):
+        self._engine = RxBeamformer(geometry)
+        self.geometry = geometry
+
+    def close(self) -> None:
+        self._engine.close()
+
+    def perform_delay(self, raw: USRawData, f_number: float, speed_of_sound_mm_per_s: float, window_type: int,
+                      window_parameter: float, out_dtype=None, out=None, stream=None) -> USRawData:
+        """RawDelay::performDelay (RawDelay.cu:555-650)."""
+        data = raw.data
+        if data.dim() != 3 or not data.is_cuda or not data.is_contiguous():
+            raise ValueError("raw.data must be a contiguous device tensor [tx][ch][s]")
+        out_dtype = out_dtype or torch.float32
+        g = self.geometry
+        shape = (g.num_rx, raw.num_received_channels, g.num_depths)
+        if out is None:
+            out = torch.empty(shape, dtype=out_dtype, device=data.device)
+        elif tuple(out.shape) != shape or out.dtype != out_dtype or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {shape}")
+        fs_out = C.c_double()
+        N.check(N.lib().supra_rawdelay_run(
+            self._engine._ctx, C.c_void_p(data.data_ptr()), _dtype_code(data.dtype), raw.num_scanlines,
+            raw.num_received_channels, raw.num_samples, float(raw.sampling_frequency), float(f_number),
+            float(speed_of_sound_mm_per_s), int(window_type), float(window_parameter), C.c_void_p(out.data_ptr()),
+            _dtype_code(out_dtype), C.byref(fs_out), C.c_void_p(_stream_handle(stream))))
+        return USRawData(out, fs_out.value, g, raw.num_elements, raw.element_layout)
+
+
+def mv_perform_rx_beamforming(raw: USRawData, sub_array_size: int = 0, temporal_smoothing: int = 3,
+                              sub_array_scaling_power: float = 1.5, compute_means: bool = False,
+                              loading_factor: float = 1.0, out_dtype=None, out=None, stream=None) -> USImage:
+    """RxBeamformerMV::performRxBeamforming (RxBeamformerMV.cu:382-614) on
+    pre-delayed data [scanline][channel][sample]; returns [sample][scanline]."""
+    data = raw.data
+    if data.dim() != 3 or not data.is_cuda or not data.is_contiguous():
+        raise ValueError("raw.data must be a contiguous device tensor [scanline][channel][sample]")
+    n_rx, n_ch, n_s = (int(v) for v in data.shape)
+    out_dtype = out_dtype or torch.float32
+    if out is None:
+        out = torch.empty((n_s, n_rx), dtype=out_dtype, device=data.device)
+    elif tuple(out.shape) != (n_s, n_rx) or out.dtype != out_dtype or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {(n_s, n_rx)}")
+    N.check(N.lib().supra_mv_run(C.c_void_p(data.data_ptr()), _dtype_code(data.dtype), n_rx, n_ch, n_s,
+                                 int(sub_array_size), int(temporal_smoothing), float(sub_array_scaling_power),
+                                 int(bool(compute_means)), float(loading_factor), C.c_void_p(out.data_ptr()),
+                                 _dtype_code(out_dtype), C.c_void_p(_stream_handle(stream))))
+    return USImage(out, (n_rx, n_s), {"imageState": "RF"})
+
+
+class RawDelayNode(BeamformingNode):
+    """RawDelayNode (RawDelayNode.cpp:25-195): parameters RawDelayNode.cpp:43-47."""
+
+    PARAMETERS = {
+        "fNumber": _Param(1.0, 0.01, 4.0),
+        "windowType": _Param("Rectangular", choices=set(WINDOW_TYPES)),
+        "windowParameter": _Param(0.0, 0.0, 10.0),
+        "speedOfSound": _Param(1540.0, 1000.0, 2000.0),
+        "outputType": _Param("float", choices=set(OUTPUT_TYPES)),
+    }
+    ALIASES = {}
+
+    def process(self, raw: USRawData, stream=None) -> USRawData:
+        """checkTypeAndDelay (RawDelayNode.cpp:108-163)."""
+        if self._engine is None or self._engine_geometry is not raw.geometry:
+            self._engine = RawDelay(raw.geometry)
+            self._engine_geometry = raw.geometry
+        out_dtype = torch.float32 if self.config["outputType"] == "float" else torch.int16
+        return self._engine.perform_delay(raw, self.config["fNumber"], self.config["speedOfSound"] * 1000.0,
+                                          WINDOW_TYPES[self.config["windowType"]], self.config["windowParameter"],
+                                          out_dtype=out_dtype, stream=stream)
+
+
+class BeamformingMVNode(BeamformingNode):
+    """BeamformingMVNode (BeamformingMVNode.cpp:25-172): parameters :42-46 plus
+    `loadingFactor` (diagonal loading = loadingFactor * trace / L; 1 = reference)."""
+
+    PARAMETERS = {
+        "subArraySize": _Param(0, 0, 64),
+        "temporalSmoothing": _Param(3, 0, 10),
+        "outputType": _Param("float", choices=set(OUTPUT_TYPES)),
+        "subArrayScalingPower": _Param(1.5, 0.5, 3.0),
+        "computeMeans": _Param(False, choices={False, True}),
+        "loadingFactor": _Param(1.0, 0.0, 1e12),
+    }
+    ALIASES = {}
+
+    def process(self, raw: USRawData, stream=None) -> USImage:
+        """checkTypeAndBeamform (BeamformingMVNode.cpp:121-163); input must be pre-delayed."""
+        out_dtype = torch.float32 if self.config["outputType"] == "float" else torch.int16
+        return mv_perform_rx_beamforming(raw, self.config["subArraySize"], self.config["temporalSmoothing"],
+                                         self.config["subArrayScalingPower"], self.config["computeMeans"],
+                                         self.config["loadingFactor"], out_dtype=out_dtype, stream=stream)
+
+
 def window_lut(window_type: int, window_parameter: float) -> np.ndarray:
     """The library's window table (supra_window_lut)."""
     out = np.zeros(N.WINDOW_NUM_ENTRIES, dtype=np.float32)

@@ -332,3 +332,84 @@ extern "C" int supra_das_run(supra_bf_ctx* ctx, int32_t sample_beamformer, const
                                 sampling_frequency_hz, f_number, speed_of_sound_mm_per_s, window_type,
                                 window_parameter, interpolate_transmits, additional_offset, out, out_dtype, stream);
 }
+
+extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype, uint32_t num_tx,
+                                  uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
+                                  double f_number, double speed_of_sound_mm_per_s, int32_t window_type,
+                                  float window_parameter, void* out, int32_t out_dtype,
+                                  double* out_sampling_frequency_hz, void* stream) {
+    clear_error();
+    if (!ctx || !rf || !out) return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: null argument");
+    if ((rf_dtype != SUPRA_INT16 && rf_dtype != SUPRA_FLOAT32) ||
+        (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
+        return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+    if (ctx->is3d)
+        return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: 3D geometries are not provided (2D pre-delay only)");
+    if (window_type < SUPRA_WINDOW_RECTANGULAR || window_type > SUPRA_WINDOW_GAUSS)
+        return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: unknown window type %d", window_type);
+    if (num_tx == 0 || num_channels == 0 || num_samples == 0)
+        return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: empty RF");
+    if (static_cast<uint64_t>(num_channels) * num_samples * (rf_dtype == SUPRA_INT16 ? 2 : 4) >= (1ull << 31))
+        return fail(SUPRA_ERR_TOO_LARGE, "rawdelay: one transmit's RF block must be smaller than 2 GiB");
+    if (!(sampling_frequency_hz > 0) || !(speed_of_sound_mm_per_s > 0) || !(f_number > 0) ||
+        !std::isfinite(sampling_frequency_hz) || !std::isfinite(speed_of_sound_mm_per_s) || !std::isfinite(f_number))
+        return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: sampling frequency, speed of sound and f-number must be positive");
+    if (!ctx->e2c.empty())
+        return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: element-to-channel maps are not used by the pre-delay");
+    for (uint32_t i = 0; i < ctx->n_rx; i++) {
+        const supra_scanline_rx& s = ctx->scan_mm[i];
+        int32_t sel, mask;
+        float w[4];
+        scanline_selection(s, &sel, &mask, w);
+        if (sel < 0) continue;
+        const supra_tx_params& t = s.tx[sel];
+        if (t.tx_scanline_idx >= num_tx)
+            return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: scanline %u uses transmit %u of %u", i,
+                        static_cast<unsigned>(t.tx_scanline_idx), num_tx);
+        if (t.last_active_element[0] > t.first_active_element[0] &&
+            static_cast<uint32_t>(t.last_active_element[0] - t.first_active_element[0]) > num_channels)
+            return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: scanline %u aperture exceeds the %u output channels", i,
+                        num_channels);
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int st = check_device(ctx);
+    if (st != SUPRA_OK) return st;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const double dt = 1.0 / sampling_frequency_hz;
+    st = ensure_dt_tables(ctx, dt, speed_of_sound_mm_per_s, hs);
+    if (st != SUPRA_OK) return st;
+    const uint32_t tile = das_depth_tile();
+    DasArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.rf = rf;
+    a.out = out;
+    a.scans = ctx->d_scans;
+    a.depths = ctx->d_depths;
+    a.ex = ctx->d_ex;
+    a.ey = ctx->d_ey;
+    a.rf_frame_stride = static_cast<uint64_t>(num_tx) * num_channels * num_samples;
+    a.out_frame_stride = static_cast<uint64_t>(ctx->n_rx) * num_channels * ctx->n_depths;
+    a.n_rx = ctx->n_rx;
+    a.n_depths = ctx->n_depths;
+    a.n_tx = num_tx;
+    a.n_ch = num_channels;
+    a.n_s = num_samples;
+    a.n_el = ctx->n_el;
+    a.layout_x = ctx->el_layout[0];
+    a.layout_y = ctx->el_layout[1];
+    a.n_frames = 1;
+    a.depth_tiles = (ctx->n_depths + tile - 1) / tile;
+    a.F = static_cast<float>(f_number);
+    a.dt = static_cast<float>(dt);
+    a.c = static_cast<float>(speed_of_sound_mm_per_s);
+    a.far_ok = 0;
+    build_window_lut(window_type, window_parameter, a.lut);
+    hipError_t e = launch_rawdelay(a, rf_dtype, out_dtype, hs);
+    if (e != hipSuccess) return hip_fail(e, "rawdelay: launch");
+    if (out_sampling_frequency_hz) {
+        // RawDelay.cu:48, :642: depth of the last sample / c / N_d.
+        const double depth = ctx->depths_mm.back();
+        *out_sampling_frequency_hz = 1 / (depth / speed_of_sound_mm_per_s / ctx->n_depths);
+    }
+    return SUPRA_OK;
+}

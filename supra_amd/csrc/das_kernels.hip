@@ -502,6 +502,98 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
     }
 }
 
+// ---- RawDelay (2D) -----------------------------------------------------------
+
+// rxDelayDTSPACEKernel + sampleDelay2D (src/SupraLib/Beamformer/RawDelay.cu:221-312,
+// 404-468): for the highest-weight transmit, every aperture element's delayed,
+// apodised sample times n/W goes to out[(s*N_ch + j)*N_d + r], j = e - first;
+// slots j >= last - first are written as 0.  Lanes = depths, so each channel
+// row of the output tile is one coalesced 256-sample store.
+template <typename RFT, typename OutT>
+__global__ __launch_bounds__(kThreads) void rawdelay2d_kernel(const DasArgs a) {
+    __shared__ float s_lut[kWindowEntries];
+    __shared__ float s_xe[kChunk2D];
+    __shared__ float s_dx[kChunk2D];
+    __shared__ float s_dx2[kChunk2D];
+    __shared__ uint32_t s_off[kChunk2D];
+
+    const Tile t = decode_tile(a);
+    for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = a.lut[i];
+    const ScanView sc = scan_view(a.scans, t.s);
+    Pixel2D p;
+    p.d = t.valid ? a.depths[t.r] : 0.0f;
+    p.aDT = (((p.d * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    p.inv = 1.0f / fminf(p.aDT, sc.max_ed_x());
+    p.px = sc.x() + sc.dir_x() * p.d;
+    const float py = sc.dir_y() * p.d;
+    p.py2 = py * py;
+    const bool vertical = (sc.dir_x() == 0.0f);
+    OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(t.frame) * a.out_frame_stride +
+                static_cast<uint64_t>(t.s) * a.n_ch * a.n_depths;
+    const int k = sc.sel();
+    int written = 0;
+    if (k >= 0) {
+        const TxDT tx = sc.tx(k);
+        const RFT* rf_tx = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(t.frame) * a.rf_frame_stride +
+                           static_cast<uint64_t>(tx.tx_idx) * a.n_ch * a.n_s;
+        const __amdgpu_buffer_rsrc_t rsrc = rf_resource(rf_tx, a.n_ch * a.n_s * sizeof(RFT));
+        const int first = tx.first_x, last = tx.last_x;
+        // pass 1: aperture weight sum and count
+        float wsum = 0.0f;
+        int nadd = 0;
+        for (int base = first; base < last; base += kChunk2D) {
+            const int cnt = min(kChunk2D, last - base);
+            __syncthreads();
+            for (int j = threadIdx.x; j < cnt; j += kThreads) s_dx[j] = a.ex[base + j] - sc.x();
+            __syncthreads();
+            for (int j = 0; j < cnt; j++) {
+                const float dx = s_dx[j];
+                if (fabsf(dx) <= p.aDT) {
+                    wsum += s_lut[window_index(dx * p.inv)];
+                    nadd++;
+                }
+            }
+        }
+        float scale = 1.0f;
+        if (nadd > 0) scale = 1.0f / wsum * static_cast<float>(nadd);
+        // pass 2: one output channel per aperture element
+        for (int base = first; base < last; base += kChunk2D) {
+            const int cnt = min(kChunk2D, last - base);
+            __syncthreads();
+            for (int j = threadIdx.x; j < cnt; j += kThreads) {
+                const int e = base + j;
+                const float xe = a.ex[e];
+                const float dx = xe - sc.x();
+                s_xe[j] = xe;
+                s_dx[j] = dx;
+                s_dx2[j] = dx * dx;
+                s_off[j] = (static_cast<uint32_t>(e) % a.n_ch) * a.n_s;
+            }
+            __syncthreads();
+            for (int j = 0; j < cnt; j++) {
+                float sample = 0.0f;
+                const float dx = s_dx[j];
+                if (fabsf(dx) <= p.aDT) {
+                    const float w = s_lut[window_index(dx * p.inv)];
+                    float q;
+                    if (vertical) {
+                        q = s_dx2[j] + p.py2;
+                    } else {
+                        const float e = s_xe[j] - p.px;
+                        q = e * e + p.py2;
+                    }
+                    const float tau = tx.t0 + (sqrt_rn(q) + p.d);
+                    gather<RFT, false>(rsrc, s_off[j], a.n_s, tau, w, sample);
+                }
+                if (t.valid) out[static_cast<uint64_t>(base - first + j) * a.n_depths + t.r] = clamp_cast<OutT>(sample * scale);
+            }
+        }
+        written = last > first ? last - first : 0;
+    }
+    if (t.valid)
+        for (uint32_t j = written; j < a.n_ch; j++) out[static_cast<uint64_t>(j) * a.n_depths + t.r] = clamp_cast<OutT>(0.0f);
+}
+
 }  // namespace
 
 template <typename RFT, typename OutT>
@@ -529,6 +621,19 @@ static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool i
 }
 
 uint32_t das_depth_tile() { return kThreads; }
+
+hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, hipStream_t stream) {
+    const uint64_t blocks = static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames;
+    dim3 grid(static_cast<uint32_t>(blocks)), block(kThreads);
+    if (rf_dtype == SUPRA_INT16) {
+        if (out_dtype == SUPRA_FLOAT32) hipLaunchKernelGGL((rawdelay2d_kernel<int16_t, float>), grid, block, 0, stream, a);
+        else hipLaunchKernelGGL((rawdelay2d_kernel<int16_t, int16_t>), grid, block, 0, stream, a);
+    } else {
+        if (out_dtype == SUPRA_FLOAT32) hipLaunchKernelGGL((rawdelay2d_kernel<float, float>), grid, block, 0, stream, a);
+        else hipLaunchKernelGGL((rawdelay2d_kernel<float, int16_t>), grid, block, 0, stream, a);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_das(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, bool testsig, bool interp_tx,
                       bool map, hipStream_t stream) {

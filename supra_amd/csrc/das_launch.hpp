@@ -14,4 +14,7 @@ uint32_t das_depth_tile();
 hipError_t launch_das(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, bool testsig, bool interp_tx,
                       bool map, hipStream_t stream);
 
+// Launch the 2D pre-delay kernel (output [frame][scanline][channel][depth]).
+hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, hipStream_t stream);
+
 }  // namespace supra_amd

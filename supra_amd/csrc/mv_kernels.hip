@@ -1,0 +1,430 @@
+// Minimum-variance receive beamforming over pre-delayed data, fused into one
+// kernel for gfx950, and the supra_mv_run entry point.
+//
+// Stands in for RxBeamformerMV::performRxBeamforming
+// (src/SupraLib/Beamformer/RxBeamformerMV.cu:382-614) and its kernels
+// (:79-378) plus the cuBLAS batched LU solve (:558-586).  The reference walks
+// every scanline and 2000-sample batch from the host (≥9 launches and 3
+// device-wide syncs per batch); here one launch covers the frame:
+//
+//   workgroup = (scanline, tile of Dt output depths), 256 lanes
+//   1. stage x[channel][tau] for the tile and its +-T halo in LDS (float), find
+//      each sample's aperture (first/last non-zero channel, :44-77), the
+//      subarray length L_t and the number of active subarrays (:79-113);
+//   2. R(tau) = sum_k (x_k - m)(x_k - m)^T for every halo sample, upper
+//      triangle packed in LDS; each lane owns 4x4 blocks and slides the two
+//      4-sample windows along k, so one LDS read feeds 8 FMAs (:151-195);
+//   3. one wave per output sample: R~ = sum over |tau - t| <= T (:197-228),
+//      diagonal loading alpha*trace/L_t (:239-305), then a Cholesky
+//      factorisation with row i of the matrix in lane i's registers, pivots
+//      and diagonal updates in FP64, triangular solves for w = R~^-1 1, an
+//      FP64 residual check with up to two steps of iterative refinement
+//      (FP32 factors, FP64 residual) for ill-conditioned samples, and the
+//      output y = sum_i (sum_k x[f+k+i]) w_i / (sum w * K^p) (:307-378).
+//
+// Numerics are FP32 with FP64 only where stated; results are compared with
+// the FP64 host oracle (per-sample relative error), not bit for bit: the
+// reference's own sums are atomics in arbitrary order and its solve is
+// cuBLAS, so its bits are not reproducible either.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+
+#include "status.hpp"
+#include "supra_bf.h"
+
+namespace supra_amd {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxTile = 64;
+
+struct MvArgs {
+    const void* x;
+    void* out;
+    uint32_t n_rx, n_ch, n_d, L, T, K;
+    uint32_t tile;          // output depths per workgroup (Dt)
+    uint32_t n_tiles;
+    uint32_t rs;            // LDS row stride of x (channels + padding)
+    uint32_t packed;        // L (L + 1) / 2
+    uint32_t waves;         // waves that solve (phase 3)
+    uint32_t off_meta, off_mean, off_r, off_wave, wave_bytes;   // LDS byte offsets
+    float p, alpha;
+    int means;
+};
+
+template <typename T> __device__ inline float in_f(T v) { return static_cast<float>(v); }
+
+template <typename T> __device__ inline T clamp_out(float x);
+template <> __device__ inline float clamp_out<float>(float x) { return fminf(fmaxf(x, -FLT_MAX), FLT_MAX); }
+template <> __device__ inline int16_t clamp_out<int16_t>(float x) {
+    return static_cast<int16_t>(fminf(fmaxf(x, -32767.0f), 32767.0f));
+}
+
+__device__ inline float wave_sum(float v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline float bcast(float v, int lane) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane)); }
+__device__ inline double bcast(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u), lane));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), lane));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+struct SampleMeta {
+    int32_t first, lt, kact, pad;
+};
+
+__device__ inline uint32_t packed_index(uint32_t i, uint32_t j, uint32_t L) {
+    // upper triangle, row-major: row i holds columns i..L-1
+    return i * L - (i * (i - 1)) / 2 + (j - i);
+}
+
+// Solve (G G^T) v = b for the wave's pixel: G's row i is in lane i's
+// registers a[] (lower part, a[i] unused; gdiag holds the diagonal), the full
+// lower factor is also in LDS (row-major, stride L+1) for the transposed solve.
+template <int LMAX>
+__device__ inline float chol_solve(const float (&a)[LMAX], float gdiag, float b, int n, int lane,
+                                   const float* lmat, int ldl) {
+    // forward: G z = b, column-oriented
+    float c = b, z = 0.0f;
+#pragma unroll
+    for (int j = 0; j < LMAX; j++) {
+        if (j < n) {
+            const float zj = bcast(c, j) / bcast(gdiag, j);
+            if (lane == j) z = zj;
+            if (lane > j && lane < n) c -= a[j] * zj;
+        }
+    }
+    // backward: G^T v = z, column j of G^T is row j of G (in LDS)
+    float v = 0.0f;
+    c = z;
+    for (int j = n - 1; j >= 0; j--) {
+        const float vj = bcast(c, j) / bcast(gdiag, j);
+        if (lane == j) v = vj;
+        if (lane < j) c -= lmat[j * ldl + lane] * vj;
+    }
+    return v;
+}
+
+template <typename InT, typename OutT, int LMAX, bool MEANS>
+__global__ __launch_bounds__(kThreads) void mv_kernel(const MvArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float* xs = reinterpret_cast<float*>(lds);
+    SampleMeta* meta = reinterpret_cast<SampleMeta*>(lds + a.off_meta);
+    float* mean = reinterpret_cast<float*>(lds + a.off_mean);
+    float* rp = reinterpret_cast<float*>(lds + a.off_r);
+
+    const uint32_t tile = blockIdx.x % a.n_tiles;
+    const uint32_t s = blockIdx.x / a.n_tiles;
+    const uint32_t t0 = tile * a.tile;
+    const uint32_t t1 = min(t0 + a.tile, a.n_d);          // output depths [t0, t1)
+    const uint32_t h_lo = t0 >= a.T ? t0 - a.T : 0;
+    const uint32_t h_hi = min(a.n_d - 1, t1 - 1 + a.T);   // halo samples [h_lo, h_hi]
+    const int H = static_cast<int>(h_hi - h_lo + 1);
+    const int L = static_cast<int>(a.L);
+    const int rs = static_cast<int>(a.rs);
+
+    // 1. stage x (zero padded to rs channels) and per-sample aperture
+    const InT* xin = static_cast<const InT*>(a.x) + static_cast<uint64_t>(s) * a.n_ch * a.n_d;
+    for (int idx = threadIdx.x; idx < H * rs; idx += kThreads) {
+        const int c = idx / H, h = idx % H;
+        xs[h * rs + c] = (c < static_cast<int>(a.n_ch)) ? in_f(xin[static_cast<uint64_t>(c) * a.n_d + h_lo + h]) : 0.0f;
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < H; h += kThreads) {
+        const float* row = xs + h * rs;
+        int f = 0, l = 0;
+        for (int c = 0; c < static_cast<int>(a.n_ch); c++)
+            if (row[c] != 0.0f) { f = c; break; }
+        for (int c = static_cast<int>(a.n_ch) - 1; c >= 0; c--)
+            if (row[c] != 0.0f) { l = c; break; }
+        SampleMeta m;
+        m.first = f;
+        m.lt = min(L, l - f + 1);
+        m.kact = min(static_cast<int>(a.K), l - f - m.lt + 2);
+        m.pad = 0;
+        meta[h] = m;
+    }
+    __syncthreads();
+    if (MEANS) {
+        for (int idx = threadIdx.x; idx < H * L; idx += kThreads) {
+            const int h = idx / L, i = idx % L;
+            const SampleMeta m = meta[h];
+            float acc = 0.0f;
+            if (i < m.lt) {
+                for (int k = 0; k < m.kact; k++) acc += xs[h * rs + m.first + k + i];
+                acc = acc / static_cast<float>(m.kact);
+            }
+            mean[h * L + i] = acc;
+        }
+        __syncthreads();
+    }
+
+    // 2. R(tau), 4x4 blocks of the upper triangle
+    const int nb = (L + 3) / 4;
+    const int nblk = nb * (nb + 1) / 2;
+    for (int item = threadIdx.x; item < H * nblk; item += kThreads) {
+        const int h = item / nblk;
+        int q = item % nblk, bi = 0;
+        while (q >= nb - bi) { q -= nb - bi; bi++; }
+        const int bj = bi + q;
+        const int i0 = 4 * bi, j0 = 4 * bj;
+        const SampleMeta m = meta[h];
+        float acc[4][4];
+#pragma unroll
+        for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) acc[ii][jj] = 0.0f;
+        if (i0 < m.lt) {
+            const float* row = xs + h * rs + m.first;
+            float ra[4], rb[4], ma[4], mb[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                ra[u] = row[i0 + u];
+                rb[u] = row[j0 + u];
+                ma[u] = (MEANS && i0 + u < L) ? mean[h * L + i0 + u] : 0.0f;
+                mb[u] = (MEANS && j0 + u < L) ? mean[h * L + j0 + u] : 0.0f;
+            }
+            for (int k = 0; k < m.kact; k++) {
+#pragma unroll
+                for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) acc[ii][jj] = fmaf(ra[ii] - ma[ii], rb[jj] - mb[jj], acc[ii][jj]);
+#pragma unroll
+                for (int u = 0; u < 3; u++) {
+                    ra[u] = ra[u + 1];
+                    rb[u] = rb[u + 1];
+                }
+                ra[3] = row[k + 1 + i0 + 3];
+                rb[3] = row[k + 1 + j0 + 3];
+            }
+        }
+        float* dst = rp + h * a.packed;
+#pragma unroll
+        for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const int i = i0 + ii, j = j0 + jj;
+                if (i < L && j < L && j >= i)
+                    dst[packed_index(i, j, L)] = (i < m.lt && j < m.lt) ? acc[ii][jj] : 0.0f;
+            }
+    }
+    __syncthreads();
+
+    // 3. one wave per output sample
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (wave >= static_cast<int>(a.waves)) return;
+    float* lmat = reinterpret_cast<float*>(lds + a.off_wave + wave * a.wave_bytes);
+    const int ldl = L + 1;
+    OutT* out = static_cast<OutT*>(a.out);
+    for (uint32_t t = t0 + wave; t < t1; t += a.waves) {
+        const int ht = static_cast<int>(t - h_lo);
+        const SampleMeta m = meta[ht];
+        const int n = m.lt;
+        const int w_lo = static_cast<int>((t >= a.T ? t - a.T : 0) - h_lo);
+        const int w_hi = static_cast<int>(min(a.n_d - 1, t + a.T) - h_lo);
+        // row `lane` of R~ (temporal sum), upper-packed source mirrored
+        float row[LMAX];
+        float diag = 0.0f;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) {
+            float v = 0.0f;
+            if (k < n && lane < n) {
+                const uint32_t pi = lane <= k ? packed_index(lane, k, L) : packed_index(k, lane, L);
+                for (int h = w_lo; h <= w_hi; h++) v += rp[h * a.packed + pi];
+            }
+            row[k] = v;
+            if (k == lane) diag = v;
+        }
+        const float trace = wave_sum(lane < n ? diag : 0.0f);
+        const float delta = ((1.0f / static_cast<float>(n)) * trace) * a.alpha;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++)
+            if (k == lane) row[k] += delta;
+        float orig[LMAX];
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) orig[k] = row[k];
+        // Cholesky, row `lane` in registers, FP64 pivots and diagonal
+        double dd = 0.0;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++)
+            if (k == lane) dd = static_cast<double>(row[k]);
+        float gdiag = 1.0f;
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < LMAX; j++) {
+            if (j < n) {
+                const double piv = bcast(dd, j);
+                ok = ok && piv > 0.0;
+                const double g = sqrt(piv);
+                const float invg = static_cast<float>(1.0 / g);
+                if (lane == j) gdiag = static_cast<float>(g);
+                if (lane > j && lane < n) row[j] *= invg;
+                if (lane < n) lmat[lane * ldl + j] = row[j];
+                wave_sync();
+                if (lane > j && lane < n) {
+                    const float lij = row[j];
+#pragma unroll
+                    for (int k = j + 1; k < LMAX; k++) {
+                        if (k < lane) row[k] = fmaf(-lij, lmat[k * ldl + j], row[k]);
+                    }
+                    dd -= static_cast<double>(lij) * static_cast<double>(lij);
+                }
+            }
+        }
+        wave_sync();
+        float w = chol_solve<LMAX>(row, gdiag, 1.0f, n, lane, lmat, ldl);
+        // FP64 residual r = R~ w - 1, up to two refinement steps
+        for (int it = 0; it < 3; it++) {
+            double r = 0.0;
+#pragma unroll
+            for (int k = 0; k < LMAX; k++)
+                if (k < n) r += static_cast<double>(orig[k]) * static_cast<double>(bcast(w, k));
+            r = (lane < n) ? r - 1.0 : 0.0;
+            const double rn = wave_sum(r * r);
+            if (!(rn > 1e-10 * n) || it == 2 || !ok) break;
+            w -= chol_solve<LMAX>(row, gdiag, static_cast<float>(r), n, lane, lmat, ldl);
+        }
+        // weights -> sample (applyWeights, RxBeamformerMV.cu:342-376)
+        float xsum = 0.0f;
+        if (lane < n)
+            for (int k = 0; k < m.kact; k++) xsum += xs[ht * rs + m.first + k + lane];
+        const float wsum = wave_sum(lane < n ? w : 0.0f);
+        const float scaling = 1.0f / (wsum * powf(static_cast<float>(m.kact), a.p));
+        float y = wave_sum(lane < n ? xsum * w * scaling : 0.0f);
+        if (fabsf(y) > 1e7f || isnan(y)) y = 0.0f;
+        if (lane == 0) out[static_cast<uint64_t>(t) * a.n_rx + s] = clamp_out<OutT>(y * static_cast<float>(a.n_ch));
+        wave_sync();
+    }
+}
+
+template <typename InT, typename OutT, int LMAX>
+hipError_t launch_typed(const MvArgs& a, size_t lds, hipStream_t stream) {
+    dim3 grid(a.n_rx * a.n_tiles), block(kThreads);
+    if (a.means) hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, true>), grid, block, lds, stream, a);
+    else hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, false>), grid, block, lds, stream, a);
+    return hipGetLastError();
+}
+
+template <typename InT, typename OutT>
+hipError_t launch_lmax(const MvArgs& a, size_t lds, hipStream_t stream) {
+    if (a.L <= 32) return launch_typed<InT, OutT, 32>(a, lds, stream);
+    return launch_typed<InT, OutT, 64>(a, lds, stream);
+}
+
+int max_lds_per_block() {
+    static std::mutex mu;
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+    std::lock_guard<std::mutex> lock(mu);
+    if (cached[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        cached[dev] = v;
+    }
+    return cached[dev];
+}
+
+// LDS layout for a tile of `tile` output depths; returns total bytes.
+size_t plan_lds(MvArgs& a, uint32_t tile) {
+    const uint32_t H = tile + 2 * a.T;
+    size_t off = static_cast<size_t>(H) * a.rs * 4;
+    a.off_meta = static_cast<uint32_t>(off);
+    off += static_cast<size_t>(H) * sizeof(SampleMeta);
+    a.off_mean = static_cast<uint32_t>(off);
+    off += a.means ? static_cast<size_t>(H) * a.L * 4 : 0;
+    a.off_r = static_cast<uint32_t>(off);
+    off += static_cast<size_t>(H) * a.packed * 4;
+    off = (off + 15) & ~static_cast<size_t>(15);
+    a.off_wave = static_cast<uint32_t>(off);
+    a.wave_bytes = static_cast<uint32_t>(((a.L * (a.L + 1) * 4) + 15) & ~15u);
+    off += static_cast<size_t>(a.waves) * a.wave_bytes;
+    return off;
+}
+
+}  // namespace
+}  // namespace supra_amd
+
+using namespace supra_amd;
+
+extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t num_scanlines, uint32_t num_channels,
+                            uint32_t num_samples, uint32_t sub_array_size, uint32_t temporal_smoothing,
+                            double sub_array_scaling_power, int32_t compute_means, double loading_factor, void* out,
+                            int32_t out_dtype, void* stream) {
+    clear_error();
+    if (!raw_delayed || !out) return fail(SUPRA_ERR_INVALID_ARG, "mv: null argument");
+    if ((in_dtype != SUPRA_INT16 && in_dtype != SUPRA_FLOAT32) ||
+        (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
+        return fail(SUPRA_ERR_UNSUPPORTED, "mv: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+    if (num_scanlines == 0 || num_channels == 0 || num_samples == 0)
+        return fail(SUPRA_ERR_INVALID_ARG, "mv: empty input");
+    const uint32_t L = sub_array_size == 0 ? num_channels / 2 : sub_array_size;
+    if (L == 0 || L > num_channels || L > 64)
+        return fail(SUPRA_ERR_UNSUPPORTED, "mv: sub-array size %u must be in [1, min(64, channels)]", L);
+    if (num_channels > 256) return fail(SUPRA_ERR_UNSUPPORTED, "mv: at most 256 channels");
+    if (temporal_smoothing > 32) return fail(SUPRA_ERR_UNSUPPORTED, "mv: temporal smoothing above 32");
+    if (!(loading_factor >= 0) || !std::isfinite(sub_array_scaling_power))
+        return fail(SUPRA_ERR_INVALID_ARG, "mv: loading factor must be >= 0");
+    MvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = raw_delayed;
+    a.out = out;
+    a.n_rx = num_scanlines;
+    a.n_ch = num_channels;
+    a.n_d = num_samples;
+    a.L = L;
+    a.T = temporal_smoothing;
+    a.K = num_channels - L + 1;
+    a.rs = num_channels + L + 4;
+    a.packed = L * (L + 1) / 2;
+    a.p = static_cast<float>(sub_array_scaling_power);
+    a.alpha = static_cast<float>(loading_factor);
+    a.means = compute_means ? 1 : 0;
+    const size_t budget = static_cast<size_t>(max_lds_per_block());
+    size_t lds = 0;
+    uint32_t tile = 0;
+    for (uint32_t waves = kWaves; waves >= 1 && tile == 0; waves /= 2) {
+        a.waves = waves;
+        for (uint32_t dt = kMaxTile; dt >= 1; dt--) {
+            if (plan_lds(a, dt) <= budget) {
+                tile = dt;
+                break;
+            }
+        }
+    }
+    if (tile == 0) return fail(SUPRA_ERR_TOO_LARGE, "mv: configuration does not fit the %zu-byte LDS", budget);
+    a.tile = std::min<uint32_t>(tile, num_samples);
+    lds = plan_lds(a, a.tile);
+    a.n_tiles = (num_samples + a.tile - 1) / a.tile;
+    if (static_cast<uint64_t>(a.n_tiles) * num_scanlines >= (1ull << 31))
+        return fail(SUPRA_ERR_TOO_LARGE, "mv: too many workgroups");
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (in_dtype == SUPRA_INT16)
+        e = (out_dtype == SUPRA_FLOAT32) ? launch_lmax<int16_t, float>(a, lds, hs) : launch_lmax<int16_t, int16_t>(a, lds, hs);
+    else
+        e = (out_dtype == SUPRA_FLOAT32) ? launch_lmax<float, float>(a, lds, hs) : launch_lmax<float, int16_t>(a, lds, hs);
+    if (e != hipSuccess) return fail(SUPRA_ERR_HIP, "mv: launch: %s", hipGetErrorString(e));
+    return SUPRA_OK;
+}

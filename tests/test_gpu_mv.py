@@ -1,0 +1,134 @@
+"""Pre-delay (bit-exact vs the FP32 oracle) and minimum variance (vs the FP64
+oracle, per-sample relative error) on the GPU through the C ABI."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from supra_amd import _native as N
+from supra_amd import workloads as W
+from supra_amd.beamformer import (BeamformingMVNode, RawDelay, RawDelayNode, USRawData,
+                                  mv_perform_rx_beamforming)
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def gpu_mv(x, **kw):
+    raw = USRawData(torch.from_numpy(np.ascontiguousarray(x)).cuda(), 1.0, None, 1, (1, 1))
+    img = mv_perform_rx_beamforming(raw, **kw)
+    torch.cuda.synchronize()
+    return img.data.cpu().numpy()
+
+
+def rel_errors(gpu, ref, floor_frac=1e-3):
+    den = np.maximum(np.abs(ref), floor_frac * np.abs(ref).max())
+    return np.abs(gpu.astype(np.float64) - ref) / den
+
+
+def test_rawdelay_bit_exact(device):
+    w = W.linear_2d()
+    g = w.geometry()
+    rf = w.rf_host()
+    rd = RawDelay(g)
+    for win, odt in [(N.WINDOW_HANN, torch.float32), (N.WINDOW_HAMMING, torch.int16)]:
+        res = rd.perform_delay(USRawData(torch.from_numpy(rf).cuda(), w.fs_hz, g), 1.0, 1.54e6, win, 0.0,
+                               out_dtype=odt)
+        torch.cuda.synchronize()
+        gpu = res.data.cpu().numpy()
+        ref = O.rawdelay(g, rf, w.fs_hz, 1.0, 1.54e6, win, out_int16=(odt == torch.int16))
+        if odt == torch.int16:
+            ref = ref.astype(np.int16)
+        same = (gpu == ref) | (np.isnan(gpu) & np.isnan(ref))
+        assert same.all(), f"{(~same).sum()} pre-delayed samples differ (window {win})"
+        assert abs(res.sampling_frequency - 1 / (40.0 / 1.54e6 / 2080)) < 1e-6
+    rd.close()
+
+
+def test_rawdelay_node(device):
+    w = W.linear_2d(num_scatterers=30)
+    g = w.geometry()
+    rf = w.rf_host()
+    node = RawDelayNode(fNumber=1.0, windowType="Hann")
+    res = node.process(USRawData(torch.from_numpy(rf).cuda(), w.fs_hz, g))
+    torch.cuda.synchronize()
+    ref = O.rawdelay(g, rf, w.fs_hz, 1.0, 1.54e6, N.WINDOW_HANN)
+    assert np.array_equal(res.data.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("alpha,T,p", [(0.01, 5, 1.5), (1.0, 0, 1.0)])
+def test_mv_plane_wave_closed_form(device, alpha, T, p):
+    rng = np.random.default_rng(4)
+    n_rx, n_ch, n_d = 16, 64, 300
+    s = (rng.uniform(1.0, 3.0, (n_rx, 1, n_d)) * rng.choice([-1.0, 1.0], (n_rx, 1, n_d))).astype(np.float32)
+    x = np.ascontiguousarray(np.broadcast_to(s, (n_rx, n_ch, n_d)))
+    out = gpu_mv(x, sub_array_size=32, temporal_smoothing=T, sub_array_scaling_power=p, loading_factor=alpha)
+    expected = (n_ch * s[:, 0, :].astype(np.float64) * 33 ** (1 - p)).T
+    assert np.abs(out / expected - 1).max() < 1e-4
+
+
+def test_mv_predelayed_frame_vs_fp64_oracle(device):
+    """The minimum-variance workload: pre-delayed linear_2d frame, L = 32, T = 5,
+    loading trace/(100 L)."""
+    w = W.linear_2d()
+    g = w.geometry()
+    rf = w.rf_host()
+    rd = RawDelay(g)
+    x = rd.perform_delay(USRawData(torch.from_numpy(rf).cuda(), w.fs_hz, g), 1.0, 1.54e6, N.WINDOW_HANN, 0.0)
+    img = mv_perform_rx_beamforming(x, 32, 5, 1.5, False, 0.01)
+    torch.cuda.synchronize()
+    gpu = img.data.cpu().numpy()
+    xh = x.data.cpu().numpy()
+    ref = O.mv(xh, 32, 5, 1.5, False, 0.01)
+    # Pre-delayed samples at -FLT_MAX (the reference's clampCast of the 0/0 of
+    # a Hann window at depth 0) overflow FP32 covariances; those samples end
+    # as NaN -> 0 in single precision (as in the reference), not in FP64.
+    huge = (np.abs(xh) > 1e19).any(axis=1)                    # [scanline, sample]
+    T = 5
+    tainted = np.zeros_like(huge)
+    for d in range(-T, T + 1):
+        tainted |= np.roll(huge, d, axis=1) if d else huge
+    tainted = tainted.T
+    assert np.all(gpu[tainted] == 0)
+    err = rel_errors(gpu[~tainted], ref[~tainted])
+    print(f"MV per-sample relative error vs FP64 oracle ({(~tainted).sum()} samples): max {err.max():.3e} "
+          f"p99 {np.quantile(err, 0.99):.3e} median {np.median(err):.3e}")
+    assert np.quantile(err, 0.99) < 1e-4
+    assert err.max() < 1e-2
+    rd.close()
+
+
+@pytest.mark.parametrize("L,T,means,n_ch", [(16, 2, False, 48), (32, 3, True, 64), (64, 5, False, 128),
+                                            (8, 0, False, 32)])
+def test_mv_random_apertures(device, L, T, means, n_ch):
+    rng = np.random.default_rng(L + T)
+    n_rx, n_d = 6, 257
+    x = rng.normal(size=(n_rx, n_ch, n_d)).astype(np.float32) * 100
+    f = rng.integers(0, n_ch // 3, (n_rx, n_d))
+    l = np.minimum(n_ch - 1, f + rng.integers(3, n_ch, (n_rx, n_d)))
+    ch = np.arange(n_ch)[None, :, None]
+    x[(ch < f[:, None, :]) | (ch > l[:, None, :])] = 0.0
+    x[:, :, 7] = 0.0                      # an all-zero sample
+    gpu = gpu_mv(x, sub_array_size=L, temporal_smoothing=T, compute_means=means, loading_factor=0.05)
+    ref = O.mv(x, L, T, 1.5, means, 0.05)
+    err = rel_errors(gpu, ref)
+    print(f"L={L} T={T} means={means}: max {err.max():.3e} p99 {np.quantile(err, 0.99):.3e}")
+    assert np.quantile(err, 0.99) < 1e-4 and err.max() < 1e-2
+    assert gpu[7].tolist() == ref[7].tolist()
+
+
+def test_mv_int16_io_and_node(device):
+    rng = np.random.default_rng(9)
+    x = (rng.normal(size=(4, 64, 200)) * 300).astype(np.int16)
+    node = BeamformingMVNode(subArraySize=32, temporalSmoothing=3, outputType="int16", loadingFactor=0.1)
+    img = node.process(USRawData(torch.from_numpy(x).cuda(), 1.0, None, 1, (1, 1)))
+    torch.cuda.synchronize()
+    gpu = img.data.cpu().numpy()
+    ref = O.mv(x, 32, 3, 1.5, False, 0.1, out_int16=True)
+    assert gpu.dtype == np.int16
+    assert np.abs(gpu.astype(np.int64) - ref.astype(np.int64)).max() <= 1
+
+
+def test_mv_rejects_large_subarray(device):
+    x = torch.zeros((2, 160, 50), device="cuda")
+    with pytest.raises(N.SupraError):
+        mv_perform_rx_beamforming(USRawData(x, 1.0, None, 1, (1, 1)), sub_array_size=80)
