@@ -122,6 +122,67 @@ def bench_2d(args, torch, frames: int = 32):
     }
 
 
+def bench_diff(args, torch, frames: int = 16):
+    """Differentiable 2D DAS: loss = ||DAS(rf; c_f) - target_f||^2 over a
+    batch of FP32 frames, c_f ~ U[1450, 1600] m/s (seed 4), targets
+    beamformed at 1540 m/s.  One step = forward + loss gradient + adjoint
+    (dL/drf and dL/dc for all frames).  Compared with the same loss written
+    in plain PyTorch (gather + autograd), one frame at a time."""
+    from supra_amd import workloads as W
+    from supra_amd.differentiable import DifferentiableDAS
+    sys.path.insert(0, str(ROOT / "tests"))
+    from torch_das_reference import PlainTorchDAS
+
+    w = W.linear_2d()
+    g = w.geometry()
+    log(f"[bench] synthesizing {frames} FP32 frames of {w.name} RF for the differentiable path")
+    rf = torch.empty((frames,) + w.rf_shape, dtype=torch.float32, device="cuda")
+    for f in range(frames):
+        W.linear_2d(scatter_seed=300 + f, noise_seed=400 + f).rf_device(
+            rf[f].data_ptr(), dtype=np.float32, stream=torch.cuda.current_stream().cuda_stream)
+    c = torch.tensor(np.random.default_rng(4).uniform(1450.0, 1600.0, frames), dtype=torch.float64, device="cuda")
+    kw = w.das_kwargs()
+    dd = DifferentiableDAS(g, w.fs_hz, kw["f_number"], kw["window_type"], kw["window_parameter"])
+    target, _ = dd.forward_raw(rf, torch.full((frames,), 1540.0, dtype=torch.float64, device="cuda"),
+                               need_norm=False)
+    tmask = target.abs() < 3e38
+
+    def step():
+        y, norm = dd.forward_raw(rf, c)
+        diff = torch.where(tmask & (y.abs() < 3e38), y - target, 0.0)
+        return dd.adjoint_raw(rf, c, norm, 2.0 * diff)
+
+    log(f"[bench] timing fused forward+adjoint, {frames} frames per step")
+    wall_ms, ev_ms = time_launches(step, max(5, args.steps), 2, torch)
+
+    plain = PlainTorchDAS(g, w.fs_hz, kw["f_number"], kw["window_type"], kw["window_parameter"],
+                          w.num_channels, w.num_samples)
+
+    def plain_step():
+        x = rf.detach().requires_grad_(True)
+        cc = c.detach().clone().requires_grad_(True)
+        for f in range(frames):
+            yp = plain(x[f], cc[f])
+            d = torch.where(tmask[f], yp - target[f], 0.0)
+            (d * d).sum().backward()
+        return x.grad, cc.grad
+
+    log("[bench] timing the plain per-frame PyTorch form")
+    p_wall, p_ev = time_launches(plain_step, 3, 1, torch)
+    b = 2 * rf.numel() * 4 + 3 * frames * g.num_rx * g.num_depths * 4
+    dd.close()
+    return {
+        "workload": f"linear_2d geometry, FP32 RF {frames}x128x64x2080, c ~ U[1450,1600] m/s, "
+                    "loss ||DAS - DAS@1540||^2, dL/drf + dL/dc",
+        "fwd_adj_ms": ev_ms,
+        "plain_torch_per_frame_ms": p_ev,
+        "speedup_vs_plain": p_ev / ev_ms,
+        "algorithmic_bytes": b,
+        "achieved_GBs": b / (ev_ms * 1e-3) / 1e9,
+        "hbm_frac": b / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+
 def cpu_baseline_3d(res, budget_s: float):
     """Host FP32 oracle on whole scanlines of the same volume, doubling the
     sample until it takes >= budget_s (or covers the volume)."""
@@ -160,6 +221,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) so that "
                          "FETCH_SIZE/WRITE_SIZE counters of a profile can be scaled")
@@ -194,6 +256,8 @@ def main() -> int:
     extra = {}
     if not args.no_2d:
         extra["linear_2d"] = bench_2d(args, torch)
+    if not args.no_diff:
+        extra["diff_2d"] = bench_diff(args, torch)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_3d(res, args.cpu_budget)

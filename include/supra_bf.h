@@ -181,6 +181,45 @@ int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer, uint32_t 
                          int32_t interpolate_transmits, int32_t additional_offset,
                          void* out, int32_t out_dtype, void* stream);
 
+/* Differentiable 2D delay-and-sum, forward.  No reference counterpart: the
+ * reference beamforms a frame with one speed of sound
+ * (RxBeamformerCuda.cu:539-663) and differentiates nothing.  This is the
+ * DelayAndSum beamformer of supra_das_run_frames with FP32 RF and image and
+ * one speed of sound per frame, read from the DEVICE array
+ * speed_of_sound_mm_per_s[num_frames] (positive, finite; the dt-space
+ * tables of every frame are built on the GPU, no host round trip).  For each
+ * frame f it produces exactly what supra_das_run does at c = c[f].
+ * norm_out (device, may be NULL): [frame][slot][scanline][depth] floats,
+ * slot count 4 with interpolate_transmits, else 1: the derivative of the
+ * pixel with respect to the slot's weighted channel sum (txWeight * n / W),
+ * or 0 where the pixel saturates; supra_das_adjoint consumes it. */
+int supra_das_forward_c(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf,
+                        uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                        double sampling_frequency_hz, double f_number,
+                        const double* speed_of_sound_mm_per_s,
+                        int32_t window_type, float window_parameter,
+                        int32_t interpolate_transmits, int32_t additional_offset,
+                        float* out, float* norm_out, void* stream);
+
+/* Adjoint of supra_das_forward_c.  No reference counterpart.  Given
+ * grad_image = dL/d(out) (device [frame][depth][scanline]) and the forward's
+ * norm_out, writes (each may be NULL):
+ *   grad_rf [frame][tx][ch][s] (device, every element written) = dL/d(rf);
+ *   grad_c  [frame] (device doubles) = dL/dc in 1/(mm/s), through the
+ *           delays: dtau/dc = -(tau - t0 - offset)/c, aperture, window and
+ *           n/W held fixed (they depend on c only through rounding).
+ * Delays are recomputed, not stored.  Arguments other than norm/grad_* must
+ * equal the forward call's.  Calls on one context share its scratch: issue
+ * them on one stream. */
+int supra_das_adjoint(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf,
+                      uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                      double sampling_frequency_hz, double f_number,
+                      const double* speed_of_sound_mm_per_s,
+                      int32_t window_type, float window_parameter,
+                      int32_t interpolate_transmits, int32_t additional_offset,
+                      const float* norm, const float* grad_image,
+                      float* grad_rf, double* grad_c, void* stream);
+
 /* RawDelay::performDelay<ChannelDataType, OutputType>
  * (src/SupraLib/Beamformer/RawDelay.cu:555-650; kernel :404-468, per-sample
  * math :221-312).  2D geometries.  For each rx scanline's highest-weight

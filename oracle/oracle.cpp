@@ -36,8 +36,8 @@ constexpr int kEntries = 64;
 
 // Static partition of [0, n) over worker threads.
 template <typename Fn>
-void parallel_for(int64_t n, int nthreads, Fn fn) {
-    if (nthreads <= 1 || n < 64) {
+void parallel_for(int64_t n, int nthreads, Fn fn, int64_t min_parallel = 64) {
+    if (nthreads <= 1 || n < min_parallel) {
         for (int64_t i = 0; i < n; i++) fn(i);
         return;
     }
@@ -593,6 +593,160 @@ void mv_scanline(const MvCall& m, uint32_t s, double* out, uint32_t dbg_sample, 
     }
 }
 
+
+// Reverse mode of the 2D DAS (das_pixel_2d) for one frame at speed of sound
+// call.c, given dL/dy.  No reference counterpart (the reference does not
+// differentiate).  The forward is linear in the RF for fixed c:
+//   y = sum_k a_k (n_k/W_k) sum_e w_e [(1-f) rf[i] + f rf[i+1]]   (i+1 < N_s)
+// so dL/drf[i] += beta w (1-f), dL/drf[i+1] += beta w f with
+// beta = dL/dy * a_k n_k / W_k.  Speed of sound: dt-space lengths scale as
+// 1/c while t0 and the offset do not, so dtau/dc = -geo/c with geo the
+// geometric part sqrt(q) + d; membership, window and n/W are held fixed.
+// Saturated pixels (NaN or beyond FLT_MAX) pass no gradient.
+// T is the arithmetic of the values (delays, fractions, sums, n/W); D the
+// arithmetic of the discrete decisions (aperture membership, window-table
+// index, floor of the delay, range tests, saturation).  T == D is the plain
+// FP32 or FP64 restatement; T = double, D = float evaluates the values in
+// double precision on the reference's single-precision decisions, which
+// separates rounding of the arithmetic from ties that float and double
+// resolve differently (e.g. window arguments that are exact multiples of
+// 1/63 at pitch multiples).
+// norm (optional): a_k n_k / W_k per [slot][scanline][depth], 0 when saturated.
+// Pinned by: tests/test_oracle_adjoint.py — the dot-product identity
+// <DAS x, y> = <x, DAS^T y> against oracle_das in FP64, dL/dc against FP64
+// central differences of oracle_das, brute force of both on a tiny geometry
+// against a pure-Python loop; the mixed mode against the FP32 and FP64 modes.
+template <typename T>
+inline uint32_t window_index_of(T rel) {
+    return static_cast<uint32_t>(vround(T(31.5) * (vmin(vmax(rel, T(-1)), T(1)) + T(1))));
+}
+
+template <typename T, typename D>
+void das_adjoint_frame(const DasCall& call, const double* grad_image, T* grad_rf, double* grad_c, T* norm_out) {
+    const supra_rx_geometry& g = *call.g;
+    const double dt_d = 1.0 / call.fs;
+    DtGeometry<T> G = to_dt_space<T>(g, dt_d, call.c);
+    DtGeometry<D> H = to_dt_space<D>(g, dt_d, call.c);
+    T lut[kEntries];
+    window_lut<T>(call.window, call.wparam, lut);
+    const D dt = static_cast<D>(dt_d), c = static_cast<D>(call.c), F = static_cast<D>(call.F);
+    const D offD = static_cast<D>(call.offset);
+    const T offT = static_cast<T>(call.offset);
+    const uint32_t n_rx = g.num_rx_scanlines, n_d = g.num_depths, n_s = call.rf.n_s, n_ch = call.rf.n_ch;
+    const int slots = call.interp_tx ? 4 : 1;
+    double dc = 0.0;
+    // One element-pixel: decisions in D, values in T.
+    struct Tap {
+        bool member, inr, edge;
+        T w, f, geo;
+        uint64_t idx;
+    };
+    auto tap = [&](uint32_t s, int k, int32_t e, uint32_t ch, D d_D, T d_T, D aDT, D inv, Tap& o) {
+        const D xeD = H.ex[e], xD = H.sx[s];
+        o.member = std::fabs(xeD - xD) <= aDT;
+        if (!o.member) return;
+        o.w = lut[window_index_of<D>((xeD - xD) * inv)];
+        const D aD = xeD - (xD + H.dirx[s] * d_D), bD = H.diry[s] * d_D;
+        D tauD = H.t0[4 * s + k] + (vsqrt(aD * aD + bD * bD) + d_D);
+        tauD = tauD + offD;
+        const int32_t i = static_cast<int32_t>(vfloor(tauD));
+        const D fD = tauD - static_cast<D>(i);
+        o.inr = static_cast<uint32_t>(i) < n_s - 1;
+        o.edge = !o.inr && static_cast<uint32_t>(i) < n_s && fD == D(0);
+        const T xeT = G.ex[e], xT = G.sx[s];
+        const T aT = xeT - (xT + G.dirx[s] * d_T), bT = G.diry[s] * d_T;
+        o.geo = vsqrt(aT * aT + bT * bT) + d_T;
+        T tauT = G.t0[4 * s + k] + o.geo;
+        tauT = tauT + offT;
+        o.f = tauT - static_cast<T>(i);
+        const supra_tx_params& tp = g.scanlines[s].tx[k];
+        o.idx = (static_cast<uint64_t>(tp.tx_scanline_idx) * n_ch + ch) * n_s + static_cast<uint32_t>(i);
+    };
+    auto channel = [&](int32_t e, uint32_t& ch) {
+        if (g.element_to_channel) {
+            if (g.element_to_channel[e] < 0) return false;
+            ch = static_cast<uint32_t>(g.element_to_channel[e]);
+        } else {
+            ch = static_cast<uint32_t>(e) % n_ch;
+        }
+        return true;
+    };
+    for (uint32_t s = 0; s < n_rx; s++) {
+        const supra_scanline_rx& sc = g.scanlines[s];
+        const int sel = select_tx(sc);
+        for (uint32_t r = 0; r < n_d; r++) {
+            const D d_D = H.depth[r];
+            const T d_T = G.depth[r];
+            const D aDT = (((d_D * dt) * c) / (D(2) * F)) / c / dt;
+            const D inv = D(1) / vmin(aDT, H.medx[s]);
+            T norm[4] = {0, 0, 0, 0};
+            D s_dec = 0;   // the forward in D, for the saturation decision
+            for (int k = 0; k < 4; k++) {
+                const bool use = call.interp_tx ? (sc.tx_weights[k] > 0.0) : (k == sel && sc.tx_weights[k] > 0.0);
+                if (!use) continue;
+                const supra_tx_params& tp = sc.tx[k];
+                T wsum = 0;
+                D sampleD = 0, wsumD = 0;
+                int nadd = 0;
+                for (int32_t e = tp.first_active_element[0]; e < tp.last_active_element[0]; e++) {
+                    uint32_t ch;
+                    if (!channel(e, ch)) continue;
+                    Tap t;
+                    tap(s, k, e, ch, d_D, d_T, aDT, inv, t);
+                    if (!t.member) continue;
+                    wsum += t.w;
+                    nadd++;
+                    const D wD = static_cast<D>(t.w);
+                    wsumD += wD;
+                    const D fD = static_cast<D>(t.f);
+                    if (t.inr)
+                        sampleD += wD * ((D(1) - fD) * rf_at<D>(call.rf, t.idx) + fD * rf_at<D>(call.rf, t.idx + 1));
+                    else if (t.edge)
+                        sampleD += wD * rf_at<D>(call.rf, t.idx);
+                }
+                const T gk = (nadd > 0) ? static_cast<T>(nadd) / wsum : T(0);
+                const T ak = call.interp_tx ? static_cast<T>(static_cast<float>(sc.tx_weights[k])) : T(1);
+                norm[k] = call.interp_tx ? ak * gk : gk;
+                const D localD = (nadd > 0) ? (sampleD / wsumD) * static_cast<D>(nadd) : D(0);
+                s_dec += call.interp_tx ? static_cast<D>(ak) * localD : localD;
+            }
+            if (!(std::fabs(s_dec) <= D(FLT_MAX)))
+                for (int k = 0; k < 4; k++) norm[k] = 0;
+            if (norm_out) {
+                for (int q = 0; q < slots; q++) {
+                    const int k = call.interp_tx ? q : sel;
+                    norm_out[(static_cast<uint64_t>(q) * n_rx + s) * n_d + r] = (k >= 0) ? norm[k] : T(0);
+                }
+            }
+            const T gy = static_cast<T>(grad_image[static_cast<uint64_t>(r) * n_rx + s]);
+            for (int k = 0; k < 4; k++) {
+                if (norm[k] == T(0)) continue;
+                const T beta = gy * norm[k];
+                const supra_tx_params& tp = sc.tx[k];
+                for (int32_t e = tp.first_active_element[0]; e < tp.last_active_element[0]; e++) {
+                    uint32_t ch;
+                    if (!channel(e, ch)) continue;
+                    Tap t;
+                    tap(s, k, e, ch, d_D, d_T, aDT, inv, t);
+                    if (!t.member) continue;
+                    const T bw = beta * t.w;
+                    if (t.inr) {
+                        if (grad_rf) {
+                            grad_rf[t.idx] += bw * (T(1) - t.f);
+                            grad_rf[t.idx + 1] += bw * t.f;
+                        }
+                        const T dv = rf_at<T>(call.rf, t.idx + 1) - rf_at<T>(call.rf, t.idx);
+                        dc += static_cast<double>((bw * dv) * t.geo);
+                    } else if (t.edge) {
+                        if (grad_rf) grad_rf[t.idx] += bw;
+                    }
+                }
+            }
+        }
+    }
+    *grad_c = -dc / call.c;
+}
+
 }  // namespace
 
 extern "C" {
@@ -661,6 +815,45 @@ int oracle_mv(const void* x, int dtype, uint32_t n_rx, uint32_t n_ch, uint32_t n
         mv_scanline(m, static_cast<uint32_t>(s), out, dbg ? debug_sample : 0xffffffffu, dbg ? debug_A : nullptr,
                     dbg ? debug_w : nullptr, dbg ? debug_meta : nullptr);
     });
+    return 0;
+}
+
+// Reverse mode of the 2D DAS for num_frames frames, one speed of sound each
+// (host array c_mm_per_s).  grad_image: double [frame][depth][scanline];
+// grad_rf: [frame][tx][ch][s] (float or double per precision, overwritten);
+// grad_c: double [frame] in 1/(mm/s); norm_out (may be NULL):
+// [frame][slot][scanline][depth].  FP32 RF.  Parallel over frames.
+// precision: 32 (FP32), 64 (FP64), 6432 (FP64 values on FP32 decisions).
+int oracle_das_adjoint(const supra_rx_geometry* g, const float* rf, uint32_t n_frames, uint32_t n_tx, uint32_t n_ch,
+                       uint32_t n_s, double fs, double f_number, const double* c_mm_per_s, int window_type,
+                       float window_parameter, int interpolate_transmits, int32_t additional_offset,
+                       const double* grad_image, int precision, void* grad_rf, double* grad_c, void* norm_out,
+                       int nthreads) {
+    if (!g || !rf || !c_mm_per_s || !grad_image || !grad_c || n_tx == 0 || n_ch == 0 || n_s == 0) return -1;
+    if (g->rx_scanline_layout[0] > 1 && g->rx_scanline_layout[1] > 1) return -3;
+    for (uint32_t i = 0; i < g->num_rx_scanlines; i++)
+        for (int k = 0; k < 4; k++)
+            if (g->scanlines[i].tx_weights[k] > 0.0 && g->scanlines[i].tx[k].tx_scanline_idx >= n_tx) return -2;
+    const uint64_t rf_frame = static_cast<uint64_t>(n_tx) * n_ch * n_s;
+    const uint64_t img = static_cast<uint64_t>(g->num_rx_scanlines) * g->num_depths;
+    const uint64_t norm_frame = img * (interpolate_transmits ? 4 : 1);
+    parallel_for(static_cast<int64_t>(n_frames), std::max(1, std::min<int>(nthreads, static_cast<int>(n_frames))),
+                 [&](int64_t f) {
+        DasCall call{g, RfView{rf + rf_frame * f, SUPRA_FLOAT32, n_tx, n_ch, n_s}, fs, f_number, c_mm_per_s[f],
+                     window_type, window_parameter, interpolate_transmits, additional_offset, 0, SUPRA_FLOAT32};
+        if (precision == 64 || precision == 6432) {
+            double* gr = grad_rf ? static_cast<double*>(grad_rf) + rf_frame * f : nullptr;
+            if (gr) std::fill(gr, gr + rf_frame, 0.0);
+            double* no = norm_out ? static_cast<double*>(norm_out) + norm_frame * f : nullptr;
+            if (precision == 64) das_adjoint_frame<double, double>(call, grad_image + img * f, gr, grad_c + f, no);
+            else das_adjoint_frame<double, float>(call, grad_image + img * f, gr, grad_c + f, no);
+        } else {
+            float* gr = grad_rf ? static_cast<float*>(grad_rf) + rf_frame * f : nullptr;
+            if (gr) std::fill(gr, gr + rf_frame, 0.0f);
+            das_adjoint_frame<float, float>(call, grad_image + img * f, gr, grad_c + f,
+                                            norm_out ? static_cast<float*>(norm_out) + norm_frame * f : nullptr);
+        }
+    }, 2);
     return 0;
 }
 

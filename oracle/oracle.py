@@ -39,6 +39,10 @@ def lib() -> C.CDLL:
         h.oracle_rawdelay.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_float, C.c_int, C.c_int,
                                       C.c_void_p, C.c_int]
+        h.oracle_das_adjoint.restype = C.c_int
+        h.oracle_das_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int32,
+                                         C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         h.oracle_mv.restype = C.c_int
         h.oracle_mv.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.c_double, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int, C.c_uint32,
@@ -82,6 +86,36 @@ def das(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_p
     if code != 0:
         raise ValueError(f"oracle_das rejected its arguments ({code})")
     return out
+
+
+def das_adjoint(geom, rf: np.ndarray, fs_hz: float, f_number: float, c_mm_per_s, window_type: int,
+                grad_image: np.ndarray, window_parameter: float = 0.0, interpolate_transmits: bool = False,
+                additional_offset: int = 0, precision: int = 64, want_rf: bool = True, want_norm: bool = False,
+                threads: int | None = None):
+    """Oracle reverse mode of the 2D DAS: rf float32 [frame][tx][ch][s], one
+    speed of sound per frame, grad_image [frame][depth][scanline].
+    Returns (grad_rf or None, grad_c [frame] in 1/(mm/s), norm or None)."""
+    rf = np.ascontiguousarray(rf, dtype=np.float32)
+    if rf.ndim == 3:
+        rf = rf[None]
+    n_f = rf.shape[0]
+    c = np.ascontiguousarray(np.broadcast_to(np.asarray(c_mm_per_s, dtype=np.float64), (n_f,)))
+    gi = np.ascontiguousarray(np.broadcast_to(np.asarray(grad_image, dtype=np.float64),
+                                              (n_f, geom.num_depths, geom.num_rx)))
+    gc = geom.to_c()
+    dt = np.float32 if precision == 32 else np.float64    # 6432: FP64 values on FP32 decisions
+    grad_rf = np.zeros(rf.shape, dtype=dt) if want_rf else None
+    grad_c = np.zeros(n_f)
+    slots = 4 if interpolate_transmits else 1
+    norm = np.zeros((n_f, slots, geom.num_rx, geom.num_depths), dtype=dt) if want_norm else None
+    code = lib().oracle_das_adjoint(C.addressof(gc), rf.ctypes.data, n_f, rf.shape[1], rf.shape[2], rf.shape[3],
+                                    fs_hz, f_number, c.ctypes.data, window_type, window_parameter,
+                                    int(interpolate_transmits), additional_offset, gi.ctypes.data, precision,
+                                    grad_rf.ctypes.data if want_rf else None, grad_c.ctypes.data,
+                                    norm.ctypes.data if want_norm else None, threads or default_threads())
+    if code != 0:
+        raise ValueError(f"oracle_das_adjoint rejected its arguments ({code})")
+    return grad_rf, grad_c, norm
 
 
 def rawdelay(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_per_s: float,

@@ -129,6 +129,12 @@ _SIGNATURES = {
     "supra_das_run_frames": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_void_p, C.c_int32, C.c_uint32,
                                        C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_int32,
                                        C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+    "supra_das_forward_c": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
+                                      C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "supra_das_adjoint": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
+                                    C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "supra_rawdelay_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_double, C.c_double, C.c_double, C.c_int32, C.c_float, C.c_void_p,
                                      C.c_int32, C.POINTER(C.c_double), C.c_void_p]),

@@ -43,7 +43,30 @@ struct supra_bf_ctx {
     std::vector<float> h_depths, h_ex, h_ey;
     double cached_dt = 0, cached_c = 0;
     bool far_ok = false;   // dt-space depths are >= 0 and non-decreasing
+    bool far_ok_mm = false;   // the same for the millimetre depths (any c)
     std::mutex mu;
+
+    // Differentiable path: the millimetre geometry on the device, the
+    // c-independent part of the scanline records, per-frame tables and
+    // scratch, grown on demand.
+    supra_scanline_rx* d_scan_mm = nullptr;
+    ScanlineDT* d_scan_fixed = nullptr;
+    float* d_depths_mm = nullptr;
+    float* d_ex_mm = nullptr;
+    void* d_frame_tables = nullptr;
+    size_t frame_tables_cap = 0;
+    void* d_beta = nullptr;
+    size_t beta_cap = 0;
+    void* d_dc_part = nullptr;
+    size_t dc_part_cap = 0;
+    // Writers of each (transmit, channel) RF line for the adjoint, cached per
+    // (transmit interpolation, num_tx, num_channels).
+    int lines_key_interp = -1;
+    uint32_t lines_key_tx = 0, lines_key_ch = 0;
+    bool lines_exclusive = false;
+    uint32_t n_zero_lines = 0;
+    uint32_t* d_zero_lines = nullptr;
+    uint32_t max_aperture = 0;   // largest last_x - first_x over scanlines and transmits
 
     ~supra_bf_ctx() {
         (void)hipFree(d_scans);
@@ -51,6 +74,14 @@ struct supra_bf_ctx {
         (void)hipFree(d_ex);
         (void)hipFree(d_ey);
         (void)hipFree(d_e2c);
+        (void)hipFree(d_scan_mm);
+        (void)hipFree(d_scan_fixed);
+        (void)hipFree(d_depths_mm);
+        (void)hipFree(d_ex_mm);
+        (void)hipFree(d_frame_tables);
+        (void)hipFree(d_beta);
+        (void)hipFree(d_dc_part);
+        (void)hipFree(d_zero_lines);
     }
 };
 
@@ -185,6 +216,10 @@ extern "C" int supra_bf_create(const supra_rx_geometry* g, supra_bf_ctx** out_ct
     ctx->ey_mm.assign(g->element_y, g->element_y + g->num_elements);
     ctx->geom_c = g->speed_of_sound_mm_per_s;
     ctx->depth_mm = g->depth;
+    ctx->far_ok_mm = true;
+    for (uint32_t i = 0; i < ctx->n_depths; i++)
+        if (!(ctx->depths_mm[i] >= 0.0f) || (i > 0 && !(ctx->depths_mm[i] >= ctx->depths_mm[i - 1])))
+            ctx->far_ok_mm = false;
     if (g->element_to_channel) {
         ctx->e2c.assign(g->element_to_channel, g->element_to_channel + g->num_elements);
         for (int32_t m : ctx->e2c) {
@@ -207,6 +242,9 @@ extern "C" int supra_bf_create(const supra_rx_geometry* g, supra_bf_ctx** out_ct
                 delete ctx;
                 return fail(SUPRA_ERR_INVALID_ARG, "create: scanline %u transmit %d aperture exceeds the array", i, k);
             }
+            if (t.last_active_element[0] > t.first_active_element[0])
+                ctx->max_aperture = std::max<uint32_t>(ctx->max_aperture,
+                                                       t.last_active_element[0] - t.first_active_element[0]);
         }
     }
     hipError_t e = hipGetDevice(&ctx->device);
@@ -410,6 +448,359 @@ extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_
         // RawDelay.cu:48, :642: depth of the last sample / c / N_d.
         const double depth = ctx->depths_mm.back();
         *out_sampling_frequency_hz = 1 / (depth / speed_of_sound_mm_per_s / ctx->n_depths);
+    }
+    return SUPRA_OK;
+}
+
+// ---- differentiable 2D DAS -----------------------------------------------------
+
+namespace {
+
+int ensure_capacity(void** p, size_t* cap, size_t bytes, const char* what) {
+    if (bytes <= *cap) return SUPRA_OK;
+    if (*p) {
+        hipError_t e = hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+        if (e != hipSuccess) return hip_fail(e, what);
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(e, what);
+    }
+    *cap = bytes;
+    return SUPRA_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
+
+// The millimetre geometry and the c-independent fields of the scanline
+// records (transmit selection, apertures, directions) on the device.
+int ensure_mm_geometry(supra_bf_ctx* ctx) {
+    if (ctx->d_scan_mm) return SUPRA_OK;
+    std::vector<ScanlineDT> fixed(ctx->n_rx);
+    for (uint32_t i = 0; i < ctx->n_rx; i++) {
+        const supra_scanline_rx& s = ctx->scan_mm[i];
+        ScanlineDT& o = fixed[i];
+        std::memset(&o, 0, sizeof(o));
+        o.dir_x = static_cast<float>(s.direction[0]);
+        o.dir_y = static_cast<float>(s.direction[1]);
+        o.dir_z = static_cast<float>(s.direction[2]);
+        scanline_selection(s, &o.sel, &o.active_mask, o.txw);
+        for (int k = 0; k < 4; k++) {
+            o.tx[k].first_x = s.tx[k].first_active_element[0];
+            o.tx[k].first_y = s.tx[k].first_active_element[1];
+            o.tx[k].last_x = s.tx[k].last_active_element[0];
+            o.tx[k].last_y = s.tx[k].last_active_element[1];
+            o.tx[k].tx_idx = s.tx[k].tx_scanline_idx;
+        }
+    }
+    hipError_t e;
+    if ((e = hipMalloc(&ctx->d_scan_mm, ctx->n_rx * sizeof(supra_scanline_rx))) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_scan_fixed, ctx->n_rx * sizeof(ScanlineDT))) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_depths_mm, ctx->n_depths * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_ex_mm, ctx->n_el * sizeof(float))) != hipSuccess ||
+        (e = hipMemcpy(ctx->d_scan_mm, ctx->scan_mm.data(), ctx->n_rx * sizeof(supra_scanline_rx),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(ctx->d_scan_fixed, fixed.data(), ctx->n_rx * sizeof(ScanlineDT), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(ctx->d_depths_mm, ctx->depths_mm.data(), ctx->n_depths * sizeof(float),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(ctx->d_ex_mm, ctx->ex_mm.data(), ctx->n_el * sizeof(float), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+        (void)hipFree(ctx->d_scan_mm);
+        ctx->d_scan_mm = nullptr;
+        return hip_fail(e, "differentiable DAS: geometry upload");
+    }
+    return SUPRA_OK;
+}
+
+struct FrameTables {
+    ScanlineDT* scans;
+    float* depths;
+    float* ex;
+    float* c;
+};
+
+// dt-space tables of every frame for the device array c (no host round trip).
+int build_frame_tables(supra_bf_ctx* ctx, uint32_t frames, const double* c_dev, double dt, hipStream_t stream,
+                       FrameTables* out) {
+    const size_t b_scans = align256(static_cast<size_t>(frames) * ctx->n_rx * sizeof(ScanlineDT));
+    const size_t b_depths = align256(static_cast<size_t>(frames) * ctx->n_depths * sizeof(float));
+    const size_t b_ex = align256(static_cast<size_t>(frames) * ctx->n_el * sizeof(float));
+    const size_t b_c = align256(static_cast<size_t>(frames) * sizeof(float));
+    int st = ensure_capacity(&ctx->d_frame_tables, &ctx->frame_tables_cap, b_scans + b_depths + b_ex + b_c,
+                             "differentiable DAS: table memory");
+    if (st != SUPRA_OK) return st;
+    char* base = static_cast<char*>(ctx->d_frame_tables);
+    out->scans = reinterpret_cast<ScanlineDT*>(base);
+    out->depths = reinterpret_cast<float*>(base + b_scans);
+    out->ex = reinterpret_cast<float*>(base + b_scans + b_depths);
+    out->c = reinterpret_cast<float*>(base + b_scans + b_depths + b_ex);
+    FrameTablesArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.scan_mm = ctx->d_scan_mm;
+    a.scan_fixed = ctx->d_scan_fixed;
+    a.depths_mm = ctx->d_depths_mm;
+    a.ex_mm = ctx->d_ex_mm;
+    a.e2c = ctx->d_e2c;
+    a.c = c_dev;
+    a.scans = out->scans;
+    a.depths = out->depths;
+    a.ex = out->ex;
+    a.frame_c = out->c;
+    a.n_rx = ctx->n_rx;
+    a.n_d = ctx->n_depths;
+    a.n_el = ctx->n_el;
+    a.n_frames = frames;
+    a.dt = dt;
+    hipError_t e = launch_frame_tables(a, stream);
+    if (e != hipSuccess) return hip_fail(e, "differentiable DAS: table launch");
+    return SUPRA_OK;
+}
+
+// Which (transmit, channel) RF lines the adjoint writes and how often: with
+// at most one writer per line the lines are stored without atomics and the
+// never-written lines are zeroed from a list.
+int ensure_line_writers(supra_bf_ctx* ctx, int interp, uint32_t num_tx, uint32_t num_channels) {
+    if (ctx->lines_key_interp == interp && ctx->lines_key_tx == num_tx && ctx->lines_key_ch == num_channels)
+        return SUPRA_OK;
+    std::vector<uint8_t> cnt(static_cast<size_t>(num_tx) * num_channels, 0);
+    for (uint32_t i = 0; i < ctx->n_rx; i++) {
+        const supra_scanline_rx& s = ctx->scan_mm[i];
+        int32_t sel, mask;
+        float w[4];
+        scanline_selection(s, &sel, &mask, w);
+        for (int k = 0; k < 4; k++) {
+            const bool used = interp ? ((mask >> k) & 1) != 0 : (k == sel);
+            if (!used) continue;
+            const supra_tx_params& t = s.tx[k];
+            for (uint32_t e = t.first_active_element[0]; e < t.last_active_element[0]; e++) {
+                uint32_t ch;
+                if (!ctx->e2c.empty()) {
+                    if (ctx->e2c[e] < 0) continue;
+                    ch = static_cast<uint32_t>(ctx->e2c[e]);
+                } else {
+                    ch = e % num_channels;
+                }
+                uint8_t& c = cnt[static_cast<size_t>(t.tx_scanline_idx) * num_channels + ch];
+                c = static_cast<uint8_t>(std::min(2, c + 1));
+            }
+        }
+    }
+    std::vector<uint32_t> zero;
+    bool exclusive = true;
+    for (size_t i = 0; i < cnt.size(); i++) {
+        if (cnt[i] > 1) exclusive = false;
+        if (cnt[i] == 0) zero.push_back(static_cast<uint32_t>(i));
+    }
+    (void)hipFree(ctx->d_zero_lines);
+    ctx->d_zero_lines = nullptr;
+    ctx->n_zero_lines = 0;
+    ctx->lines_key_interp = -1;
+    if (exclusive && !zero.empty()) {
+        hipError_t e;
+        if ((e = hipMalloc(&ctx->d_zero_lines, zero.size() * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMemcpy(ctx->d_zero_lines, zero.data(), zero.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+                hipSuccess)
+            return hip_fail(e, "adjoint: line list upload");
+        ctx->n_zero_lines = static_cast<uint32_t>(zero.size());
+    }
+    ctx->lines_exclusive = exclusive;
+    ctx->lines_key_interp = interp;
+    ctx->lines_key_tx = num_tx;
+    ctx->lines_key_ch = num_channels;
+    return SUPRA_OK;
+}
+
+// Argument checks shared by the differentiable entry points.
+int check_diff_call(const supra_bf_ctx* ctx, const char* what, uint32_t num_frames, uint32_t num_tx,
+                    uint32_t num_channels, uint32_t num_samples, double fs, double f_number, int32_t window_type,
+                    int32_t interpolate_transmits) {
+    if (ctx->is3d) return fail(SUPRA_ERR_UNSUPPORTED, "%s: 2D geometries only", what);
+    if (num_frames > 65535) return fail(SUPRA_ERR_TOO_LARGE, "%s: at most 65535 frames per call", what);
+    if (window_type < SUPRA_WINDOW_RECTANGULAR || window_type > SUPRA_WINDOW_GAUSS)
+        return fail(SUPRA_ERR_INVALID_ARG, "%s: unknown window type %d", what, window_type);
+    if (num_tx == 0 || num_channels == 0 || num_samples == 0)
+        return fail(SUPRA_ERR_INVALID_ARG, "%s: empty RF (num_tx=%u num_channels=%u num_samples=%u)", what, num_tx,
+                    num_channels, num_samples);
+    if (static_cast<uint64_t>(num_channels) * num_samples * 4 >= (1ull << 31))
+        return fail(SUPRA_ERR_TOO_LARGE, "%s: one transmit's RF block must be smaller than 2 GiB", what);
+    if (!(fs > 0) || !(f_number > 0) || !std::isfinite(fs) || !std::isfinite(f_number))
+        return fail(SUPRA_ERR_INVALID_ARG, "%s: sampling frequency and f-number must be positive", what);
+    if (!ctx->e2c.empty() && ctx->max_channel_in_map >= num_channels)
+        return fail(SUPRA_ERR_INVALID_ARG, "%s: element map addresses channel %u of %u", what,
+                    ctx->max_channel_in_map, num_channels);
+    for (uint32_t i = 0; i < ctx->n_rx; i++) {
+        const supra_scanline_rx& s = ctx->scan_mm[i];
+        int32_t sel, mask;
+        float w[4];
+        scanline_selection(s, &sel, &mask, w);
+        for (int k = 0; k < 4; k++) {
+            bool used = interpolate_transmits ? ((mask >> k) & 1) : (k == sel);
+            if (used && s.tx[k].tx_scanline_idx >= num_tx)
+                return fail(SUPRA_ERR_INVALID_ARG, "%s: scanline %u uses transmit %u of %u", what, i,
+                            static_cast<unsigned>(s.tx[k].tx_scanline_idx), num_tx);
+        }
+    }
+    return SUPRA_OK;
+}
+
+}  // namespace
+
+extern "C" int supra_das_forward_c(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf, uint32_t num_tx,
+                                   uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
+                                   double f_number, const double* speed_of_sound_mm_per_s, int32_t window_type,
+                                   float window_parameter, int32_t interpolate_transmits, int32_t additional_offset,
+                                   float* out, float* norm_out, void* stream) {
+    clear_error();
+    if (!ctx || !rf || !out || !speed_of_sound_mm_per_s)
+        return fail(SUPRA_ERR_INVALID_ARG, "das forward: null argument");
+    if (num_frames == 0) return SUPRA_OK;
+    int st = check_diff_call(ctx, "das forward", num_frames, num_tx, num_channels, num_samples,
+                             sampling_frequency_hz, f_number, window_type, interpolate_transmits);
+    if (st != SUPRA_OK) return st;
+    const uint64_t tile = das_depth_tile();
+    const uint64_t blocks = ((ctx->n_depths + tile - 1) / tile) * ctx->n_rx * static_cast<uint64_t>(num_frames);
+    if (blocks >= (1ull << 31)) return fail(SUPRA_ERR_TOO_LARGE, "das forward: too many frames for one launch");
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if ((st = check_device(ctx)) != SUPRA_OK) return st;
+    if ((st = ensure_mm_geometry(ctx)) != SUPRA_OK) return st;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const double dt = 1.0 / sampling_frequency_hz;
+    FrameTables ft;
+    if ((st = build_frame_tables(ctx, num_frames, speed_of_sound_mm_per_s, dt, hs, &ft)) != SUPRA_OK) return st;
+
+    DasArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.rf = rf;
+    a.out = out;
+    a.scans = ft.scans;
+    a.depths = ft.depths;
+    a.ex = ft.ex;
+    a.ey = ft.ex;   // unused in 2D
+    a.e2c = ctx->d_e2c;
+    a.rf_frame_stride = static_cast<uint64_t>(num_tx) * num_channels * num_samples;
+    a.out_frame_stride = static_cast<uint64_t>(ctx->n_rx) * ctx->n_depths;
+    a.n_rx = ctx->n_rx;
+    a.n_depths = ctx->n_depths;
+    a.n_tx = num_tx;
+    a.n_ch = num_channels;
+    a.n_s = num_samples;
+    a.n_el = ctx->n_el;
+    a.layout_x = ctx->el_layout[0];
+    a.layout_y = ctx->el_layout[1];
+    a.n_frames = num_frames;
+    a.depth_tiles = static_cast<uint32_t>((ctx->n_depths + tile - 1) / tile);
+    a.F = static_cast<float>(f_number);
+    a.dt = static_cast<float>(dt);
+    a.c = 0.0f;
+    a.offset = static_cast<float>(additional_offset);
+    a.far_ok = ctx->far_ok_mm ? 1 : 0;
+    a.scan_frame_stride = ctx->n_rx;
+    a.depth_frame_stride = ctx->n_depths;
+    a.el_frame_stride = ctx->n_el;
+    a.frame_c = ft.c;
+    a.norm_out = norm_out;
+    build_window_lut(window_type, window_parameter, a.lut);
+    hipError_t e = launch_das(a, SUPRA_FLOAT32, SUPRA_FLOAT32, false, false, interpolate_transmits != 0,
+                              ctx->d_e2c != nullptr, hs);
+    if (e != hipSuccess) return hip_fail(e, "das forward: launch");
+    return SUPRA_OK;
+}
+
+extern "C" int supra_das_adjoint(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf, uint32_t num_tx,
+                                 uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
+                                 double f_number, const double* speed_of_sound_mm_per_s, int32_t window_type,
+                                 float window_parameter, int32_t interpolate_transmits, int32_t additional_offset,
+                                 const float* norm, const float* grad_image, float* grad_rf, double* grad_c,
+                                 void* stream) {
+    clear_error();
+    if (!ctx || !rf || !speed_of_sound_mm_per_s || !norm || !grad_image)
+        return fail(SUPRA_ERR_INVALID_ARG, "das adjoint: null argument");
+    if (num_frames == 0 || (!grad_rf && !grad_c)) return SUPRA_OK;
+    int st = check_diff_call(ctx, "das adjoint", num_frames, num_tx, num_channels, num_samples,
+                             sampling_frequency_hz, f_number, window_type, interpolate_transmits);
+    if (st != SUPRA_OK) return st;
+    if (ctx->n_rx > 65535u * 64u || ctx->n_depths > 65535u * 64u)
+        return fail(SUPRA_ERR_TOO_LARGE, "das adjoint: image too large");
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if ((st = check_device(ctx)) != SUPRA_OK) return st;
+    // Waves (aperture elements) per workgroup: each holds one RF line in LDS.
+    int dev_lds = 0;
+    hipError_t e = hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "das adjoint: device attribute");
+    uint32_t waves = 4;
+    while (waves > 1 && das_adjoint_lds_bytes(waves, num_samples) > 65536) waves /= 2;
+    if (das_adjoint_lds_bytes(waves, num_samples) > static_cast<size_t>(dev_lds))
+        return fail(SUPRA_ERR_TOO_LARGE, "das adjoint: %u samples per channel do not fit in LDS", num_samples);
+    const uint32_t slots = interpolate_transmits ? 4u : 1u;
+    const uint32_t egroups = std::max(1u, (ctx->max_aperture + waves - 1) / waves);
+    const uint64_t blocks = static_cast<uint64_t>(num_frames) * ctx->n_rx * slots * egroups;
+    if (blocks >= (1ull << 31)) return fail(SUPRA_ERR_TOO_LARGE, "das adjoint: too many frames for one launch");
+
+    if ((st = ensure_mm_geometry(ctx)) != SUPRA_OK) return st;
+    if ((st = ensure_line_writers(ctx, interpolate_transmits != 0, num_tx, num_channels)) != SUPRA_OK) return st;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const double dt = 1.0 / sampling_frequency_hz;
+    FrameTables ft;
+    if ((st = build_frame_tables(ctx, num_frames, speed_of_sound_mm_per_s, dt, hs, &ft)) != SUPRA_OK) return st;
+    const size_t n_beta = static_cast<size_t>(num_frames) * slots * ctx->n_rx * ctx->n_depths;
+    if ((st = ensure_capacity(&ctx->d_beta, &ctx->beta_cap, n_beta * sizeof(float), "das adjoint: scratch")) !=
+        SUPRA_OK)
+        return st;
+    const size_t per_frame = static_cast<size_t>(ctx->n_rx) * slots * egroups * waves;
+    if (grad_c && (st = ensure_capacity(&ctx->d_dc_part, &ctx->dc_part_cap, per_frame * num_frames * sizeof(double),
+                                        "das adjoint: scratch")) != SUPRA_OK)
+        return st;
+
+    const uint64_t rf_frame = static_cast<uint64_t>(num_tx) * num_channels * num_samples;
+    if (grad_rf) {
+        if (!ctx->lines_exclusive) e = launch_fill_zero(grad_rf, rf_frame * num_frames, hs);
+        else e = launch_zero_lines(grad_rf, ctx->d_zero_lines, ctx->n_zero_lines, num_samples, rf_frame, num_frames,
+                                   hs);
+        if (e != hipSuccess) return hip_fail(e, "das adjoint: zero launch");
+    }
+    e = launch_beta(grad_image, norm, static_cast<float*>(ctx->d_beta), ctx->n_rx, ctx->n_depths, slots, num_frames,
+                    hs);
+    if (e != hipSuccess) return hip_fail(e, "das adjoint: beta launch");
+
+    AdjointArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.rf = rf;
+    a.beta = static_cast<const float*>(ctx->d_beta);
+    a.scans = ft.scans;
+    a.depths = ft.depths;
+    a.ex = ft.ex;
+    a.frame_c = ft.c;
+    a.e2c = ctx->d_e2c;
+    a.grad_rf = grad_rf;
+    a.dc_part = grad_c ? static_cast<double*>(ctx->d_dc_part) : nullptr;
+    a.rf_frame_stride = rf_frame;
+    a.scan_frame_stride = ctx->n_rx;
+    a.depth_frame_stride = ctx->n_depths;
+    a.el_frame_stride = ctx->n_el;
+    a.n_rx = ctx->n_rx;
+    a.n_depths = ctx->n_depths;
+    a.n_ch = num_channels;
+    a.n_s = num_samples;
+    a.n_slots = slots;
+    a.egroups = egroups;
+    a.waves = waves;
+    a.F = static_cast<float>(f_number);
+    a.dt = static_cast<float>(dt);
+    a.offset = static_cast<float>(additional_offset);
+    a.exclusive = ctx->lines_exclusive ? 1 : 0;
+    a.interp_tx = interpolate_transmits ? 1 : 0;
+    build_window_lut(window_type, window_parameter, a.lut);
+    if ((e = launch_das_adjoint(a, num_frames, hs)) != hipSuccess) return hip_fail(e, "das adjoint: launch");
+    if (grad_c) {
+        e = launch_dc_reduce(static_cast<const double*>(ctx->d_dc_part), static_cast<uint32_t>(per_frame),
+                             speed_of_sound_mm_per_s, grad_c, num_frames, hs);
+        if (e != hipSuccess) return hip_fail(e, "das adjoint: reduce launch");
     }
     return SUPRA_OK;
 }

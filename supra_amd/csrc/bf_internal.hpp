@@ -63,6 +63,56 @@ struct DasArgs {
     float offset;               // (float) additionalOffset
     int32_t max_aperture;       // unused (0)
     int32_t far_ok;             // depth table is non-negative and non-decreasing
+    // Per-frame geometry (one speed of sound per frame, 2D): strides between
+    // the frames' tables (0: all frames share one table) and the per-frame c
+    // as a float (nullptr: every frame uses c).
+    uint32_t scan_frame_stride;     // ScanlineDT records
+    uint32_t depth_frame_stride;    // floats
+    uint32_t el_frame_stride;       // floats
+    const float* frame_c;
+    // Optional second output of the 2D kernel (float images only): per
+    // frame, transmit slot, scanline and depth, d(pixel)/d(channel sum of the
+    // slot) = txWeight * n / W, or 0 where the pixel saturates (NaN or
+    // beyond FLT_MAX).  Slots: 4 with transmit interpolation, else 1.
+    float* norm_out;
+    float lut[kWindowEntries];
+};
+
+// Per-frame dt-space tables built on the GPU from the millimetre geometry for
+// a device array of speeds of sound: the same double-precision expressions
+// and float casts as the host-side table builder.
+struct FrameTablesArgs {
+    const supra_scanline_rx* scan_mm;   // [n_rx]
+    const ScanlineDT* scan_fixed;       // [n_rx] c-independent fields (sel, weights, apertures, t0)
+    const float* depths_mm;             // [n_d]
+    const float* ex_mm;                 // [n_el]
+    const int32_t* e2c;                 // or nullptr
+    const double* c;                    // [n_frames] mm/s (device)
+    ScanlineDT* scans;                  // [frame][n_rx]
+    float* depths;                      // [frame][n_d]
+    float* ex;                          // [frame][n_el]
+    float* frame_c;                     // [frame]
+    uint32_t n_rx, n_d, n_el, n_frames;
+    double dt;
+};
+
+// The adjoint of the 2D DAS with respect to the RF and the speed of sound.
+struct AdjointArgs {
+    const float* rf;                // [frame][tx][ch][s]
+    const float* beta;              // [frame][slot][scanline][depth]: dL/dy * norm
+    const ScanlineDT* scans;        // per frame (scan_frame_stride)
+    const float* depths;            // per frame (depth_frame_stride)
+    const float* ex;                // per frame (el_frame_stride)
+    const float* frame_c;           // [frame] float c
+    const int32_t* e2c;             // or nullptr
+    float* grad_rf;                 // [frame][tx][ch][s] or nullptr
+    double* dc_part;                // [frame][blocks_per_frame * waves] partial sums, or nullptr
+    uint64_t rf_frame_stride;
+    uint32_t scan_frame_stride, depth_frame_stride, el_frame_stride;
+    uint32_t n_rx, n_depths, n_ch, n_s, n_slots, egroups, waves;
+    float F, dt, offset;
+    int32_t exclusive;              // every written line has exactly one writer
+    int32_t interp_tx;
     float lut[kWindowEntries];
 };
 

@@ -37,6 +37,7 @@
 #include <cstdint>
 
 #include "bf_internal.hpp"
+#include "das_device.hpp"
 #include "das_launch.hpp"
 
 namespace supra_amd {
@@ -48,111 +49,6 @@ constexpr int kChunk2D = 512;     // aperture elements staged per LDS pass (2D)
 constexpr int kChunk3D = 512;     // aperture elements staged per LDS pass (3D)
 constexpr int kW2Stride = kWindowEntries + 1;   // padded rows: spread LDS banks
 constexpr uint32_t kNotConnected = 0xffffffffu;
-constexpr int kBufferDword3 = 0x00020000;       // gfx9 raw buffer descriptor word 3
-
-// clampCast (src/SupraLib/utilities/cudaUtility.h:111-147): fmaxf/fminf keep
-// the non-NaN operand, so a NaN input lands on the lower limit.
-template <typename T> __device__ inline T clamp_cast(float x);
-template <> __device__ inline float clamp_cast<float>(float x) {
-    return fminf(fmaxf(x, -FLT_MAX), FLT_MAX);
-}
-template <> __device__ inline int16_t clamp_cast<int16_t>(float x) {
-    return static_cast<int16_t>(fminf(fmaxf(x, -32767.0f), 32767.0f));
-}
-
-// WindowFunctionGpu::get (WindowFunction.h:58-64): roundf of a value in
-// [0, 63], computed as truncation plus a half-up step (identical to roundf
-// for non-negative arguments).
-__device__ inline int window_index(float rel) {
-    const float x = 31.5f * (fminf(fmaxf(rel, -1.0f), 1.0f) + 1.0f);
-    const uint32_t t = static_cast<uint32_t>(x);
-    return static_cast<int>(t + ((x - static_cast<float>(t)) >= 0.5f ? 1u : 0u));
-}
-
-// Correctly rounded single-precision square root for x == 0 or normal x
-// (every argument here is a sum of squares of dt-space lengths, so it is
-// either exactly zero or far above 2^-96).  v_sqrt_f32 is within one ulp;
-// one residual test on each neighbour picks the correctly rounded value
-// (the correction LLVM emits for sqrtf, without the denormal rescale and the
-// class test that this domain does not need).
-__device__ inline float sqrt_rn(float x) {
-    float s = __builtin_amdgcn_sqrtf(x);
-    const uint32_t si = __float_as_uint(s);
-    const float down = __uint_as_float(si - 1u);
-    const float up = __uint_as_float(si + 1u);
-    const float r_down = fmaf(-down, s, x);
-    const float r_up = fmaf(-up, s, x);
-    s = (r_down <= 0.0f) ? down : s;
-    s = (r_up > 0.0f) ? up : s;
-    return s;
-}
-
-// Workgroup-uniform tables are read through the constant address space so
-// that they arrive in SGPRs (scalar loads); the buffer descriptor built from
-// them is then uniform and the gathers need no per-lane descriptor loop.
-typedef const __attribute__((address_space(4))) uint32_t* UniformWords;
-
-// Field accessors of one ScanlineDT record through scalar loads (no copy of
-// the record into registers or private memory; transmit k may be dynamic).
-struct ScanView {
-    UniformWords w;
-    __device__ float f(size_t byte_offset) const { return __uint_as_float(w[byte_offset / 4]); }
-    __device__ uint32_t u(size_t byte_offset) const { return w[byte_offset / 4]; }
-    __device__ float x() const { return f(offsetof(ScanlineDT, x)); }
-    __device__ float z() const { return f(offsetof(ScanlineDT, z)); }
-    __device__ float dir_x() const { return f(offsetof(ScanlineDT, dir_x)); }
-    __device__ float dir_y() const { return f(offsetof(ScanlineDT, dir_y)); }
-    __device__ float dir_z() const { return f(offsetof(ScanlineDT, dir_z)); }
-    __device__ float max_ed_x() const { return f(offsetof(ScanlineDT, max_ed_x)); }
-    __device__ float max_ed_y() const { return f(offsetof(ScanlineDT, max_ed_y)); }
-    __device__ int32_t sel() const { return static_cast<int32_t>(u(offsetof(ScanlineDT, sel))); }
-    __device__ int32_t active_mask() const { return static_cast<int32_t>(u(offsetof(ScanlineDT, active_mask))); }
-    __device__ float txw(int k) const { return __uint_as_float(w[offsetof(ScanlineDT, txw) / 4 + k]); }
-    __device__ TxDT tx(int k) const {
-        const size_t b = (offsetof(ScanlineDT, tx) + static_cast<size_t>(k) * sizeof(TxDT)) / 4;
-        TxDT t;
-        const uint32_t w0 = w[b], w1 = w[b + 1];
-        t.first_x = static_cast<uint16_t>(w0 & 0xffffu);
-        t.first_y = static_cast<uint16_t>(w0 >> 16);
-        t.last_x = static_cast<uint16_t>(w1 & 0xffffu);
-        t.last_y = static_cast<uint16_t>(w1 >> 16);
-        t.tx_idx = w[b + 2];
-        t.t0 = __uint_as_float(w[b + 3]);
-        t.far_thr = __uint_as_float(w[b + 4]);
-        return t;
-    }
-};
-
-__device__ inline ScanView scan_view(const ScanlineDT* table, uint32_t s) {
-    return ScanView{(UniformWords)(table + s)};
-}
-
-__device__ inline float load_uniform_float(const float* table, uint32_t i) {
-    return __uint_as_float(((UniformWords)table)[i]);
-}
-
-// readfirstlane returns int: each half goes back through uint32_t before it
-// is widened, or a low word with bit 31 set would sign-extend over the high word.
-__device__ inline uint32_t uniform_u32(uint32_t v) {
-    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
-}
-
-__device__ inline __amdgpu_buffer_rsrc_t rf_resource(const void* base, uint32_t bytes) {
-    const uint64_t v = reinterpret_cast<uint64_t>(base);
-    const uint64_t u = (static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(v >> 32))) << 32) |
-                       static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(v)));
-    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(u), 0, static_cast<int>(uniform_u32(bytes)),
-                                             kBufferDword3);
-}
-
-template <typename T> __device__ inline float load_rf(__amdgpu_buffer_rsrc_t r, uint32_t idx);
-template <> __device__ inline float load_rf<int16_t>(__amdgpu_buffer_rsrc_t r, uint32_t idx) {
-    return static_cast<float>(static_cast<int16_t>(
-        __builtin_amdgcn_raw_buffer_load_b16(r, static_cast<int>(idx * 2u), 0, 0)));
-}
-template <> __device__ inline float load_rf<float>(__amdgpu_buffer_rsrc_t r, uint32_t idx) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(idx * 4u), 0, 0));
-}
 
 // Linear interpolation of one channel at delay tau, with the reference's
 // range rules (RxSampleBeamformerDelayAndSum.h:174-187 for 2D, :147-161 for
@@ -265,10 +161,16 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
     const Tile t = decode_tile(a);
     for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = a.lut[i];
 
-    const ScanView sc = scan_view(a.scans, t.s);
+    // Per-frame tables (strides are 0 when every frame shares one geometry).
+    const ScanlineDT* scans = a.scans + static_cast<size_t>(t.frame) * a.scan_frame_stride;
+    const float* depths = a.depths + static_cast<size_t>(t.frame) * a.depth_frame_stride;
+    const float* ex = a.ex + static_cast<size_t>(t.frame) * a.el_frame_stride;
+    const float c = a.frame_c ? load_uniform_float(a.frame_c, t.frame) : a.c;
+
+    const ScanView sc = scan_view(scans, t.s);
     Pixel2D p;
-    p.d = t.valid ? a.depths[t.r] : 0.0f;
-    p.aDT = (((p.d * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    p.d = t.valid ? depths[t.r] : 0.0f;
+    p.aDT = (((p.d * a.dt) * c) / (2.0f * a.F)) / c / a.dt;
     p.inv = 1.0f / fminf(p.aDT, sc.max_ed_x());
     p.px = sc.x() + sc.dir_x() * p.d;
     const float py = sc.dir_y() * p.d;
@@ -276,20 +178,27 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
     const bool vertical = (sc.dir_x() == 0.0f);
     // aDT at the tile's first depth (the smallest one: the host enables the
     // far-field path only for non-negative, non-decreasing depth tables)
-    const float d0 = load_uniform_float(a.depths, t.r0);
-    const float aDT0 = (((d0 * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    const float d0 = load_uniform_float(depths, t.r0);
+    const float aDT0 = (((d0 * a.dt) * c) / (2.0f * a.F)) / c / a.dt;
     const float inv_far = 1.0f / sc.max_ed_x();
+    const size_t norm_slot_stride = static_cast<size_t>(a.n_rx) * a.n_depths;
+    float* norm_px = a.norm_out ? a.norm_out + (static_cast<size_t>(t.frame) * (INTERP_TX ? 4 : 1) * a.n_rx + t.s) *
+                                                a.n_depths + t.r
+                             : nullptr;
 
     const RFT* rf_frame = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(t.frame) * a.rf_frame_stride;
     float s_interp = 0.0f;
 
     for (int k = 0; k < 4; k++) {
         const bool use = INTERP_TX ? ((sc.active_mask() >> k) & 1) != 0 : (k == sc.sel());
-        if (!use) continue;   // uniform over the workgroup
+        if (!use) {   // uniform over the workgroup
+            if (INTERP_TX && norm_px && t.valid) norm_px[k * norm_slot_stride] = 0.0f;
+            continue;
+        }
         const TxDT tx = sc.tx(k);
         float s_local;
         if (TESTSIG) {
-            s_local = test_signal_2d(sc.x(), sc.dir_x(), sc.dir_y(), p.d, a.dt, a.c);
+            s_local = test_signal_2d(sc.x(), sc.dir_x(), sc.dir_y(), p.d, a.dt, c);
         } else {
             const RFT* rf_tx = rf_frame + static_cast<uint64_t>(tx.tx_idx) * a.n_ch * a.n_s;
             const __amdgpu_buffer_rsrc_t rsrc = rf_resource(rf_tx, a.n_ch * a.n_s * sizeof(RFT));
@@ -309,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
                     } else {
                         ch = static_cast<uint32_t>(e) % a.n_ch;
                     }
-                    const float xe = a.ex[e];
+                    const float xe = ex[e];
                     const float dx = xe - sc.x();
                     s_xe[j] = xe;
                     s_dx[j] = dx;
@@ -335,6 +244,10 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
                 }
             }
             s_local = (nadd > 0) ? (sample / wsum) * static_cast<float>(nadd) : 0.0f;
+            if (norm_px && t.valid) {
+                const float g = (nadd > 0) ? static_cast<float>(nadd) / wsum : 0.0f;
+                norm_px[(INTERP_TX ? k : 0) * norm_slot_stride] = INTERP_TX ? sc.txw(k) * g : g;
+            }
         }
         if (INTERP_TX) s_interp += sc.txw(k) * s_local;
         else s_interp += s_local;
@@ -342,6 +255,10 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
     if (t.valid) {
         OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(t.frame) * a.out_frame_stride;
         out[static_cast<uint64_t>(t.r) * a.n_rx + t.s] = clamp_cast<OutT>(s_interp);
+        // A saturated pixel (NaN or beyond FLT_MAX) is constant under small
+        // changes of its inputs: zero derivative.
+        if (norm_px && !(fabsf(s_interp) <= FLT_MAX))
+            for (int k = 0; k < (INTERP_TX ? 4 : 1); k++) norm_px[k * norm_slot_stride] = 0.0f;
     }
 }
 

@@ -17,4 +17,16 @@ hipError_t launch_das(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, 
 // Launch the 2D pre-delay kernel (output [frame][scanline][channel][depth]).
 hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, hipStream_t stream);
 
+// Differentiable 2D DAS (das_adjoint.hip).
+hipError_t launch_frame_tables(const FrameTablesArgs& a, hipStream_t stream);
+hipError_t launch_beta(const float* grad_image, const float* norm, float* beta, uint32_t n_rx, uint32_t n_d,
+                       uint32_t slots, uint32_t frames, hipStream_t stream);
+size_t das_adjoint_lds_bytes(uint32_t waves, uint32_t n_s);
+hipError_t launch_das_adjoint(const AdjointArgs& a, uint32_t frames, hipStream_t stream);
+hipError_t launch_dc_reduce(const double* part, uint32_t per_frame, const double* c, double* grad_c, uint32_t frames,
+                            hipStream_t stream);
+hipError_t launch_zero_lines(float* grad_rf, const uint32_t* lines, uint32_t n_lines, uint32_t n_s,
+                             uint64_t frame_stride, uint32_t frames, hipStream_t stream);
+hipError_t launch_fill_zero(float* p, uint64_t n, hipStream_t stream);
+
 }  // namespace supra_amd
