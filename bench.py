@@ -29,6 +29,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 METRIC = "beamformed samples/s & % HBM peak (DAS 2D/3D); MV pixels/s; fwd+adj ms"
 
 
@@ -183,6 +184,67 @@ def bench_diff(args, torch, frames: int = 16):
     }
 
 
+MV_L, MV_T, MV_P, MV_ALPHA = 32, 5, 1.5, 0.01
+
+
+def mv_flops_per_pixel(n_ch: int, L: int, T: int) -> float:
+    """The reference's algorithm per output sample: K full L x L outer
+    products, the temporal sum of 2T+1 matrices, an L^3/3 factorisation, two
+    triangular solves, the subarray sums and the weights."""
+    K = n_ch - L + 1
+    return 2 * K * L * L + (2 * T + 1) * L * L + L ** 3 / 3 + 2 * L * L + K * L + 6 * L
+
+
+def bench_mv(args, torch):
+    """Minimum variance on the 2D workload: RawDelay (F#1, Hann) of one int16
+    frame, then L = 32, T = 5, loading trace/(100 L) over 128 x 2080 samples
+    (266,240 32x32 systems) in one launch.  Compared with the same algorithm
+    in plain PyTorch on the whole frame (gather, batched matmul,
+    torch.linalg.solve)."""
+    from supra_amd import _native as N
+    from supra_amd import workloads as W
+    from supra_amd.beamformer import RawDelay, USRawData, mv_perform_rx_beamforming
+    sys.path.insert(0, str(ROOT / "tests"))
+    from torch_mv_reference import plain_mv
+
+    w = W.linear_2d()
+    g = w.geometry()
+    rf = torch.empty(w.rf_shape, dtype=torch.int16, device="cuda")
+    w.rf_device(rf.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
+    rd = RawDelay(g)
+    raw = USRawData(rf, w.fs_hz, g)
+    kw = w.das_kwargs()
+    delayed = rd.perform_delay(raw, kw["f_number"], kw["speed_of_sound_mm_per_s"], N.WINDOW_HANN, 0.0)
+    out = torch.empty((g.num_depths, g.num_rx), dtype=torch.float32, device="cuda")
+
+    def mv_step():
+        mv_perform_rx_beamforming(delayed, MV_L, MV_T, MV_P, False, MV_ALPHA, out=out)
+
+    def delay_step():
+        rd.perform_delay(raw, kw["f_number"], kw["speed_of_sound_mm_per_s"], N.WINDOW_HANN, 0.0, out=delayed.data)
+
+    log("[bench] timing minimum variance (fused kernel) and the pre-delay")
+    _, mv_ms = time_launches(mv_step, max(5, args.steps), 2, torch)
+    _, rd_ms = time_launches(delay_step, max(5, args.steps), 2, torch)
+    log("[bench] timing the plain PyTorch minimum variance")
+    _, plain_ms = time_launches(lambda: plain_mv(delayed.data, MV_L, MV_T, MV_P, MV_ALPHA), 3, 1, torch)
+    pixels = g.num_rx * g.num_depths
+    flops = mv_flops_per_pixel(w.num_channels, MV_L, MV_T) * pixels
+    rd.close()
+    return {
+        "workload": "linear_2d frame -> RawDelay (F#1 Hann) -> MV L=32 T=5 loading trace/(100L), "
+                    "128x2080 samples",
+        "mv_pixels_per_s": pixels / (mv_ms * 1e-3),
+        "mv_ms": mv_ms,
+        "rawdelay_ms": rd_ms,
+        "plain_torch_mv_ms": plain_ms,
+        "speedup_vs_plain": plain_ms / mv_ms,
+        "algorithmic_flop": flops,
+        "achieved_TFLOPs": flops / (mv_ms * 1e-3) / 1e12,
+        "fp32_frac": flops / (mv_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+    }
+
+
 def cpu_baseline_3d(res, budget_s: float):
     """Host FP32 oracle on whole scanlines of the same volume, doubling the
     sample until it takes >= budget_s (or covers the volume)."""
@@ -222,6 +284,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-mv", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) so that "
                          "FETCH_SIZE/WRITE_SIZE counters of a profile can be scaled")
@@ -258,6 +321,8 @@ def main() -> int:
         extra["linear_2d"] = bench_2d(args, torch)
     if not args.no_diff:
         extra["diff_2d"] = bench_diff(args, torch)
+    if not args.no_mv:
+        extra["mv_2d"] = bench_mv(args, torch)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_3d(res, args.cpu_budget)

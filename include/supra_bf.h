@@ -248,6 +248,22 @@ int supra_mv_run(const void* raw_delayed, int32_t in_dtype,
                  double sub_array_scaling_power, int32_t compute_means, double loading_factor,
                  void* out, int32_t out_dtype, void* stream);
 
+/* The pre-delay of supra_rawdelay_run feeding supra_mv_run, fused into one
+ * launch: the pre-delayed samples are computed in LDS per workgroup (tile of
+ * depths plus the temporal-smoothing halo) and never written to memory.
+ * No reference counterpart (the reference chains RawDelayNode and
+ * BeamformingMVNode through a device buffer, RawDelay.cu:555-650 then
+ * RxBeamformerMV.cu:382-614).  Output [num_depths][num_rx_scanlines],
+ * bit-identical to the two-step path with a float intermediate; the
+ * minimum-variance input has num_channels channels. */
+int supra_mv_das_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype,
+                     uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                     double sampling_frequency_hz, double f_number, double speed_of_sound_mm_per_s,
+                     int32_t window_type, float window_parameter,
+                     uint32_t sub_array_size, uint32_t temporal_smoothing,
+                     double sub_array_scaling_power, int32_t compute_means, double loading_factor,
+                     void* out, int32_t out_dtype, void* stream);
+
 /* Window LUT of the reference (WindowFunction.cpp:18-47, WindowFunction.h:122-138),
  * computed on the host: SUPRA_WINDOW_NUM_ENTRIES floats, max-normalised. */
 int supra_window_lut(int32_t window_type, float window_parameter, float* lut_out);

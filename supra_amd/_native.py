@@ -140,6 +140,10 @@ _SIGNATURES = {
                                      C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
     "supra_mv_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                C.c_double, C.c_int32, C.c_double, C.c_void_p, C.c_int32, C.c_void_p]),
+    "supra_mv_das_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_double, C.c_double, C.c_double, C.c_int32, C.c_float, C.c_uint32,
+                                   C.c_uint32, C.c_double, C.c_int32, C.c_double, C.c_void_p, C.c_int32,
+                                   C.c_void_p]),
     "supra_window_lut": (C.c_int, [C.c_int32, C.c_float, C.POINTER(C.c_float)]),
     "supra_last_error": (C.c_char_p, []),
     "supra_version": (C.c_char_p, []),

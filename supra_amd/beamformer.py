@@ -276,6 +276,31 @@ def mv_perform_rx_beamforming(raw: USRawData, sub_array_size: int = 0, temporal_
     return USImage(out, (n_rx, n_s), {"imageState": "RF"})
 
 
+def mv_das_perform_rx_beamforming(engine: "RxBeamformer", raw: USRawData, f_number: float,
+                                  speed_of_sound_mm_per_s: float, window_type: int, window_parameter: float,
+                                  sub_array_size: int = 0, temporal_smoothing: int = 3,
+                                  sub_array_scaling_power: float = 1.5, compute_means: bool = False,
+                                  loading_factor: float = 1.0, out_dtype=None, out=None, stream=None) -> USImage:
+    """RawDelay::performDelay followed by RxBeamformerMV::performRxBeamforming
+    in one fused launch (supra_mv_das_run): RF [tx][ch][s] -> [depth][scanline]."""
+    data = raw.data
+    if data.dim() != 3 or not data.is_cuda or not data.is_contiguous():
+        raise ValueError("raw.data must be a contiguous device tensor [tx][ch][s]")
+    g = engine.geometry
+    out_dtype = out_dtype or torch.float32
+    if out is None:
+        out = torch.empty((g.num_depths, g.num_rx), dtype=out_dtype, device=data.device)
+    elif tuple(out.shape) != (g.num_depths, g.num_rx) or out.dtype != out_dtype or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {(g.num_depths, g.num_rx)}")
+    N.check(N.lib().supra_mv_das_run(
+        engine._ctx, C.c_void_p(data.data_ptr()), _dtype_code(data.dtype), raw.num_scanlines,
+        raw.num_received_channels, raw.num_samples, float(raw.sampling_frequency), float(f_number),
+        float(speed_of_sound_mm_per_s), int(window_type), float(window_parameter), int(sub_array_size),
+        int(temporal_smoothing), float(sub_array_scaling_power), int(bool(compute_means)), float(loading_factor),
+        C.c_void_p(out.data_ptr()), _dtype_code(out_dtype), C.c_void_p(_stream_handle(stream))))
+    return USImage(out, (g.num_rx, g.num_depths), {"imageState": "RF"})
+
+
 class RawDelayNode(BeamformingNode):
     """RawDelayNode (RawDelayNode.cpp:25-195): parameters RawDelayNode.cpp:43-47."""
 

@@ -18,6 +18,7 @@
 
 #include "bf_internal.hpp"
 #include "das_launch.hpp"
+#include "mv_launch.hpp"
 #include "status.hpp"
 #include "supra_bf.h"
 
@@ -371,29 +372,25 @@ extern "C" int supra_das_run(supra_bf_ctx* ctx, int32_t sample_beamformer, const
                                 window_parameter, interpolate_transmits, additional_offset, out, out_dtype, stream);
 }
 
-extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype, uint32_t num_tx,
-                                  uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
-                                  double f_number, double speed_of_sound_mm_per_s, int32_t window_type,
-                                  float window_parameter, void* out, int32_t out_dtype,
-                                  double* out_sampling_frequency_hz, void* stream) {
-    clear_error();
-    if (!ctx || !rf || !out) return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: null argument");
-    if ((rf_dtype != SUPRA_INT16 && rf_dtype != SUPRA_FLOAT32) ||
-        (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
-        return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
-    if (ctx->is3d)
-        return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: 3D geometries are not provided (2D pre-delay only)");
+namespace {
+
+// Checks of the pre-delay entry points (RawDelay::performDelay preconditions).
+int check_rawdelay_call(const supra_bf_ctx* ctx, const char* what, int32_t rf_dtype, uint32_t num_tx,
+                        uint32_t num_channels, uint32_t num_samples, double fs, double f_number, double c,
+                        int32_t window_type) {
+    if (rf_dtype != SUPRA_INT16 && rf_dtype != SUPRA_FLOAT32)
+        return fail(SUPRA_ERR_UNSUPPORTED, "%s: element types must be SUPRA_INT16 or SUPRA_FLOAT32", what);
+    if (ctx->is3d) return fail(SUPRA_ERR_UNSUPPORTED, "%s: 3D geometries are not provided (2D pre-delay only)", what);
     if (window_type < SUPRA_WINDOW_RECTANGULAR || window_type > SUPRA_WINDOW_GAUSS)
-        return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: unknown window type %d", window_type);
-    if (num_tx == 0 || num_channels == 0 || num_samples == 0)
-        return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: empty RF");
+        return fail(SUPRA_ERR_INVALID_ARG, "%s: unknown window type %d", what, window_type);
+    if (num_tx == 0 || num_channels == 0 || num_samples == 0) return fail(SUPRA_ERR_INVALID_ARG, "%s: empty RF", what);
     if (static_cast<uint64_t>(num_channels) * num_samples * (rf_dtype == SUPRA_INT16 ? 2 : 4) >= (1ull << 31))
-        return fail(SUPRA_ERR_TOO_LARGE, "rawdelay: one transmit's RF block must be smaller than 2 GiB");
-    if (!(sampling_frequency_hz > 0) || !(speed_of_sound_mm_per_s > 0) || !(f_number > 0) ||
-        !std::isfinite(sampling_frequency_hz) || !std::isfinite(speed_of_sound_mm_per_s) || !std::isfinite(f_number))
-        return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: sampling frequency, speed of sound and f-number must be positive");
+        return fail(SUPRA_ERR_TOO_LARGE, "%s: one transmit's RF block must be smaller than 2 GiB", what);
+    if (!(fs > 0) || !(c > 0) || !(f_number > 0) || !std::isfinite(fs) || !std::isfinite(c) || !std::isfinite(f_number))
+        return fail(SUPRA_ERR_INVALID_ARG, "%s: sampling frequency, speed of sound and f-number must be positive",
+                    what);
     if (!ctx->e2c.empty())
-        return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: element-to-channel maps are not used by the pre-delay");
+        return fail(SUPRA_ERR_UNSUPPORTED, "%s: element-to-channel maps are not used by the pre-delay", what);
     for (uint32_t i = 0; i < ctx->n_rx; i++) {
         const supra_scanline_rx& s = ctx->scan_mm[i];
         int32_t sel, mask;
@@ -402,25 +399,23 @@ extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_
         if (sel < 0) continue;
         const supra_tx_params& t = s.tx[sel];
         if (t.tx_scanline_idx >= num_tx)
-            return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: scanline %u uses transmit %u of %u", i,
+            return fail(SUPRA_ERR_INVALID_ARG, "%s: scanline %u uses transmit %u of %u", what, i,
                         static_cast<unsigned>(t.tx_scanline_idx), num_tx);
         if (t.last_active_element[0] > t.first_active_element[0] &&
             static_cast<uint32_t>(t.last_active_element[0] - t.first_active_element[0]) > num_channels)
-            return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: scanline %u aperture exceeds the %u output channels", i,
+            return fail(SUPRA_ERR_INVALID_ARG, "%s: scanline %u aperture exceeds the %u output channels", what, i,
                         num_channels);
     }
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    int st = check_device(ctx);
-    if (st != SUPRA_OK) return st;
-    hipStream_t hs = static_cast<hipStream_t>(stream);
-    const double dt = 1.0 / sampling_frequency_hz;
-    st = ensure_dt_tables(ctx, dt, speed_of_sound_mm_per_s, hs);
-    if (st != SUPRA_OK) return st;
-    const uint32_t tile = das_depth_tile();
+    return SUPRA_OK;
+}
+
+// DasArgs of a one-frame 2D pre-delay (tables must be current).
+DasArgs rawdelay_args(const supra_bf_ctx* ctx, const void* rf, uint32_t num_tx, uint32_t num_channels,
+                      uint32_t num_samples, double dt, double f_number, double c, int32_t window_type,
+                      float window_parameter) {
     DasArgs a;
     std::memset(&a, 0, sizeof(a));
     a.rf = rf;
-    a.out = out;
     a.scans = ctx->d_scans;
     a.depths = ctx->d_depths;
     a.ex = ctx->d_ex;
@@ -436,12 +431,37 @@ extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_
     a.layout_x = ctx->el_layout[0];
     a.layout_y = ctx->el_layout[1];
     a.n_frames = 1;
-    a.depth_tiles = (ctx->n_depths + tile - 1) / tile;
+    a.depth_tiles = (ctx->n_depths + das_depth_tile() - 1) / das_depth_tile();
     a.F = static_cast<float>(f_number);
     a.dt = static_cast<float>(dt);
-    a.c = static_cast<float>(speed_of_sound_mm_per_s);
+    a.c = static_cast<float>(c);
     a.far_ok = 0;
     build_window_lut(window_type, window_parameter, a.lut);
+    return a;
+}
+
+}  // namespace
+
+extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype, uint32_t num_tx,
+                                  uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
+                                  double f_number, double speed_of_sound_mm_per_s, int32_t window_type,
+                                  float window_parameter, void* out, int32_t out_dtype,
+                                  double* out_sampling_frequency_hz, void* stream) {
+    clear_error();
+    if (!ctx || !rf || !out) return fail(SUPRA_ERR_INVALID_ARG, "rawdelay: null argument");
+    if (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32)
+        return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+    int st = check_rawdelay_call(ctx, "rawdelay", rf_dtype, num_tx, num_channels, num_samples, sampling_frequency_hz,
+                                 f_number, speed_of_sound_mm_per_s, window_type);
+    if (st != SUPRA_OK) return st;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if ((st = check_device(ctx)) != SUPRA_OK) return st;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const double dt = 1.0 / sampling_frequency_hz;
+    if ((st = ensure_dt_tables(ctx, dt, speed_of_sound_mm_per_s, hs)) != SUPRA_OK) return st;
+    DasArgs a = rawdelay_args(ctx, rf, num_tx, num_channels, num_samples, dt, f_number, speed_of_sound_mm_per_s,
+                              window_type, window_parameter);
+    a.out = out;
     hipError_t e = launch_rawdelay(a, rf_dtype, out_dtype, hs);
     if (e != hipSuccess) return hip_fail(e, "rawdelay: launch");
     if (out_sampling_frequency_hz) {
@@ -450,6 +470,30 @@ extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_
         *out_sampling_frequency_hz = 1 / (depth / speed_of_sound_mm_per_s / ctx->n_depths);
     }
     return SUPRA_OK;
+}
+
+extern "C" int supra_mv_das_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype, uint32_t num_tx,
+                                uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
+                                double f_number, double speed_of_sound_mm_per_s, int32_t window_type,
+                                float window_parameter, uint32_t sub_array_size, uint32_t temporal_smoothing,
+                                double sub_array_scaling_power, int32_t compute_means, double loading_factor,
+                                void* out, int32_t out_dtype, void* stream) {
+    clear_error();
+    if (!ctx || !rf || !out) return fail(SUPRA_ERR_INVALID_ARG, "mv: null argument");
+    if (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32)
+        return fail(SUPRA_ERR_UNSUPPORTED, "mv: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+    int st = check_rawdelay_call(ctx, "mv", rf_dtype, num_tx, num_channels, num_samples, sampling_frequency_hz,
+                                 f_number, speed_of_sound_mm_per_s, window_type);
+    if (st != SUPRA_OK) return st;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if ((st = check_device(ctx)) != SUPRA_OK) return st;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const double dt = 1.0 / sampling_frequency_hz;
+    if ((st = ensure_dt_tables(ctx, dt, speed_of_sound_mm_per_s, hs)) != SUPRA_OK) return st;
+    const DasArgs g = rawdelay_args(ctx, rf, num_tx, num_channels, num_samples, dt, f_number,
+                                    speed_of_sound_mm_per_s, window_type, window_parameter);
+    return launch_mv_fused(g, rf_dtype, sub_array_size, temporal_smoothing, sub_array_scaling_power, compute_means,
+                           loading_factor, out, out_dtype, hs);
 }
 
 // ---- differentiable 2D DAS -----------------------------------------------------

