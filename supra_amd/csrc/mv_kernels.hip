@@ -95,7 +95,8 @@ __device__ inline double bcast(double v, int lane) {
 }
 
 struct SampleMeta {
-    int32_t first, lt, kact, pad;
+    int32_t first, lt, kact;
+    int32_t bad;    // a non-finite channel value: every window holding it ends as NaN -> 0
 };
 
 __device__ inline uint32_t packed_index(uint32_t i, uint32_t j, uint32_t L) {
@@ -103,29 +104,54 @@ __device__ inline uint32_t packed_index(uint32_t i, uint32_t j, uint32_t L) {
     return i * L - (i * (i - 1)) / 2 + (j - i);
 }
 
-// Solve (G G^T) v = b for the wave's pixel: G's row i is in lane i's
-// registers a[] (lower part, a[i] unused; gdiag holds the diagonal), the full
-// lower factor is also in LDS (row-major, stride L+1) for the transposed solve.
-template <int LMAX>
-__device__ inline float chol_solve(const float (&a)[LMAX], float gdiag, float b, int n, int lane,
-                                   const float* lmat, int ldl) {
+// Sub-wave helpers: with PPW == 2 pixels per wave, lanes [0, 32) and
+// [32, 64) each hold one 32 x 32 system (row i in lane i of the half).
+template <int PPW>
+__device__ inline float sub_bcast(float v, int j, int half) {
+    if (PPW == 1) return bcast(v, j);
+    const float a0 = bcast(v, j), a1 = bcast(v, j + 32);
+    return half ? a1 : a0;
+}
+template <int PPW>
+__device__ inline double sub_bcast(double v, int j, int half) {
+    if (PPW == 1) return bcast(v, j);
+    const double a0 = bcast(v, j), a1 = bcast(v, j + 32);
+    return half ? a1 : a0;
+}
+template <int PPW>
+__device__ inline float sub_sum(float v) {
+    for (int m = 32 / PPW; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+template <int PPW>
+__device__ inline double sub_sum(double v) {
+    for (int m = 32 / PPW; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Solve (G G^T) v = b for the sub-wave's pixel: G's row i is in lane i's
+// registers a[] (strictly lower part), ginv is lane i's 1/G[i][i]; the
+// strictly lower factor is also in LDS (row-major packed, G[r][c] at
+// r (r + 1) / 2 + c) for the transposed solve.  Lanes i >= n carry no data.
+template <int LMAX, int PPW>
+__device__ inline float chol_solve(const float (&a)[LMAX], float ginv, float b, int n, int li, int half,
+                                   const float* lmat) {
     // forward: G z = b, column-oriented
     float c = b, z = 0.0f;
 #pragma unroll
     for (int j = 0; j < LMAX; j++) {
-        if (j < n) {
-            const float zj = bcast(c, j) / bcast(gdiag, j);
-            if (lane == j) z = zj;
-            if (lane > j && lane < n) c -= a[j] * zj;
-        }
+        const float zj = sub_bcast<PPW>(c * ginv, j, half);
+        if (li == j) z = zj;
+        if (li > j && li < n && j < n) c = fmaf(-a[j], zj, c);
     }
     // backward: G^T v = z, column j of G^T is row j of G (in LDS)
     float v = 0.0f;
     c = z;
-    for (int j = n - 1; j >= 0; j--) {
-        const float vj = bcast(c, j) / bcast(gdiag, j);
-        if (lane == j) v = vj;
-        if (lane < j) c -= lmat[j * ldl + lane] * vj;
+#pragma unroll
+    for (int j = LMAX - 1; j >= 0; j--) {
+        const float vj = sub_bcast<PPW>(c * ginv, j, half);
+        if (li == j) v = vj;
+        if (li < j && j < n) c = fmaf(-lmat[(j * (j + 1)) / 2 + li], vj, c);
     }
     return v;
 }
@@ -266,7 +292,7 @@ __device__ inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, ui
         m.first = f;
         m.lt = min(L, l - f + 1);
         m.kact = min(static_cast<int>(a.K), l - f - m.lt + 2);
-        m.pad = 0;
+        m.bad = 0;
         meta[h] = m;
     }
     __syncthreads();
@@ -301,123 +327,178 @@ __device__ inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, ui
             for (int jj = 0; jj < 4; jj++) acc[ii][jj] = 0.0f;
         if (i0 < m.lt) {
             const float* row = xs + h * rs + m.first;
-            float ra[4], rb[4], ma[4], mb[4];
+            float ma[4], mb[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                ra[u] = row[i0 + u];
-                rb[u] = row[j0 + u];
                 ma[u] = (MEANS && i0 + u < L) ? mean[h * L + i0 + u] : 0.0f;
                 mb[u] = (MEANS && j0 + u < L) ? mean[h * L + j0 + u] : 0.0f;
             }
-            for (int k = 0; k < m.kact; k++) {
+            // k in steps of 4 with 7-sample windows (no register shifting);
+            // every entry still accumulates k = 0, 1, 2, ... in order
+            const int kact = m.kact;
+            int k = 0;
+            for (; k + 4 <= kact; k += 4) {
+                float va[7], vb[7];
 #pragma unroll
-                for (int ii = 0; ii < 4; ii++)
-#pragma unroll
-                    for (int jj = 0; jj < 4; jj++) acc[ii][jj] = fmaf(ra[ii] - ma[ii], rb[jj] - mb[jj], acc[ii][jj]);
-#pragma unroll
-                for (int u = 0; u < 3; u++) {
-                    ra[u] = ra[u + 1];
-                    rb[u] = rb[u + 1];
+                for (int u = 0; u < 7; u++) {
+                    va[u] = row[k + i0 + u];
+                    vb[u] = row[k + j0 + u];
                 }
-                ra[3] = row[k + 1 + i0 + 3];
-                rb[3] = row[k + 1 + j0 + 3];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int ii = 0; ii < 4; ii++) {
+                        const float av = va[q + ii] - ma[ii];
+#pragma unroll
+                        for (int jj = 0; jj < 4; jj++) acc[ii][jj] = fmaf(av, vb[q + jj] - mb[jj], acc[ii][jj]);
+                    }
+            }
+            for (; k < kact; k++) {
+#pragma unroll
+                for (int ii = 0; ii < 4; ii++) {
+                    const float av = row[k + i0 + ii] - ma[ii];
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) acc[ii][jj] = fmaf(av, row[k + j0 + jj] - mb[jj], acc[ii][jj]);
+                }
             }
         }
         float* dst = rp + h * a.packed;
+        bool nonfinite = false;
 #pragma unroll
         for (int ii = 0; ii < 4; ii++)
 #pragma unroll
             for (int jj = 0; jj < 4; jj++) {
                 const int i = i0 + ii, j = j0 + jj;
-                if (i < L && j < L && j >= i)
-                    dst[packed_index(i, j, L)] = (i < m.lt && j < m.lt) ? acc[ii][jj] : 0.0f;
+                if (i < L && j < L && j >= i) {
+                    const float v = (i < m.lt && j < m.lt) ? acc[ii][jj] : 0.0f;
+                    nonfinite |= !(fabsf(v) <= FLT_MAX);
+                    dst[packed_index(i, j, L)] = v;
+                }
             }
+        // An overflowed or NaN covariance (e.g. from a -FLT_MAX pre-delayed
+        // sample) makes every window holding the sample NaN in single
+        // precision, hence output 0; it is kept out of the window sums.
+        if (nonfinite) meta[h].bad = 1;
     }
     __syncthreads();
 
-    // 3. one wave per output sample
+    const int packed = static_cast<int>(a.packed);
+
+    // 3. per output sample: one wave (PPW == 1) or one half-wave (PPW == 2)
+    constexpr int PPW = (LMAX <= 32) ? 2 : 1;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (wave >= static_cast<int>(a.waves)) return;   // (no barrier follows in this tile)
-    float* lmat = reinterpret_cast<float*>(lds + a.off_wave + wave * a.wave_bytes);
-    const int ldl = L + 1;
+    const int half = (PPW == 2) ? lane / 32 : 0;
+    const int li = (PPW == 2) ? lane % 32 : lane;
+    // strictly lower factor, row-major packed: G[r][c] at r (r + 1) / 2 + c
+    float* lmat = reinterpret_cast<float*>(lds + a.off_wave + (wave * PPW + half) * a.wave_bytes);
+    const int lrow = (li * (li + 1)) / 2;
     OutT* out = static_cast<OutT*>(a.out);
-    for (uint32_t t = t0 + wave; t < t1; t += a.waves) {
+    for (uint32_t tb = t0 + wave * PPW; tb < t1; tb += a.waves * PPW) {
+        const bool valid = tb + half < t1;
+        const uint32_t t = valid ? tb + half : t1 - 1;    // a spare half repeats the last sample
         const int ht = static_cast<int>(t - h_lo);
         const SampleMeta m = meta[ht];
         const int n = m.lt;
         const int w_lo = static_cast<int>((t >= a.T ? t - a.T : 0) - h_lo);
         const int w_hi = static_cast<int>(min(a.n_d - 1, t + a.T) - h_lo);
-        // row `lane` of R~ (temporal sum), upper-packed source mirrored
+        // row `li` of R~ = sum of R(tau) over the window (upper-packed source
+        // mirrored); the window loop is outside so that each step issues the
+        // row's LDS loads back to back.  A window holding a non-finite
+        // covariance (see phase 2) ends as NaN in single precision: output 0.
         float row[LMAX];
-        float diag = 0.0f;
 #pragma unroll
-        for (int k = 0; k < LMAX; k++) {
-            float v = 0.0f;
-            if (k < n && lane < n) {
-                const uint32_t pi = lane <= k ? packed_index(lane, k, L) : packed_index(k, lane, L);
-                for (int h = w_lo; h <= w_hi; h++) v += rp[h * a.packed + pi];
-            }
-            row[k] = v;
-            if (k == lane) diag = v;
-        }
-        const float trace = wave_sum(lane < n ? diag : 0.0f);
-        const float delta = ((1.0f / static_cast<float>(n)) * trace) * a.alpha;
+        for (int k = 0; k < LMAX; k++) row[k] = 0.0f;
+        const int row_base = li * L - (li * (li - 1)) / 2 - li;   // packed_index(li, k) = row_base + k, k >= li
+        int bad = 0;
+        for (int h = w_lo; h <= w_hi; h++) {
+            const SampleMeta mh = meta[h];
+            bad |= mh.bad;
+            if (mh.bad) continue;
+            const float* src = rp + h * packed;
 #pragma unroll
-        for (int k = 0; k < LMAX; k++)
-            if (k == lane) row[k] += delta;
-        float orig[LMAX];
-#pragma unroll
-        for (int k = 0; k < LMAX; k++) orig[k] = row[k];
-        // Cholesky, row `lane` in registers, FP64 pivots and diagonal
-        double dd = 0.0;
-#pragma unroll
-        for (int k = 0; k < LMAX; k++)
-            if (k == lane) dd = static_cast<double>(row[k]);
-        float gdiag = 1.0f;
-        bool ok = true;
-#pragma unroll
-        for (int j = 0; j < LMAX; j++) {
-            if (j < n) {
-                const double piv = bcast(dd, j);
-                ok = ok && piv > 0.0;
-                const double g = sqrt(piv);
-                const float invg = static_cast<float>(1.0 / g);
-                if (lane == j) gdiag = static_cast<float>(g);
-                if (lane > j && lane < n) row[j] *= invg;
-                if (lane < n) lmat[lane * ldl + j] = row[j];
-                wave_sync();
-                if (lane > j && lane < n) {
-                    const float lij = row[j];
-#pragma unroll
-                    for (int k = j + 1; k < LMAX; k++) {
-                        if (k < lane) row[k] = fmaf(-lij, lmat[k * ldl + j], row[k]);
-                    }
-                    dd -= static_cast<double>(lij) * static_cast<double>(lij);
+            for (int k = 0; k < LMAX; k++) {
+                if (k < n && li < n) {
+                    const int pi = (k >= li) ? row_base + k : (k * L - (k * (k - 1)) / 2 - k + li);
+                    row[k] += src[pi];
                 }
             }
         }
+        float diag = 0.0f;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++)
+            if (k == li) diag = row[k];
+        const float trace = sub_sum<PPW>(li < n ? diag : 0.0f);
+        const float delta = ((1.0f / static_cast<float>(n)) * trace) * a.alpha;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++)
+            if (k == li) row[k] += delta;
+        float orig[LMAX];
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) orig[k] = row[k];
+        // Cholesky, row `li` in registers, FP64 pivots and diagonal
+        double dd = 0.0;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++)
+            if (k == li) dd = static_cast<double>(row[k]);
+        float ginv = 1.0f;
+        bool ok = true;
+        // steps beyond the larger system of the wave are skipped (uniform branch)
+        const int nmax = (PPW == 1) ? n : max(__builtin_amdgcn_readlane(n, 0), __builtin_amdgcn_readlane(n, 32));
+#pragma unroll
+        for (int j = 0; j < LMAX; j++) {
+            if (j >= nmax) break;
+            const double piv = sub_bcast<PPW>(dd, j, half);
+            if (j < n) {
+                ok = ok && piv > 0.0;
+                // 1/sqrt(pivot) in FP64: hardware estimate and one Newton step
+                double r = __builtin_amdgcn_rsq(piv);
+                r = r * (1.5 - (0.5 * piv) * (r * r));
+                const float invg = static_cast<float>(r);
+                if (li == j) ginv = invg;
+                if (li > j && li < n) row[j] *= invg;
+                if (li > j && li < n) lmat[lrow + j] = row[j];
+            }
+            wave_sync();
+            // Branch-free update of the lanes below the pivot (others take a
+            // zero multiplier).  Entries k >= li of row li are its diagonal
+            // (tracked in dd) and upper part, which are never read again, so
+            // they are updated unmasked.
+            const float nlij = (j < n && li > j && li < n) ? -row[j] : 0.0f;
+#pragma unroll
+            for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, lmat[(k * (k + 1)) / 2 + j], row[k]);
+            dd -= static_cast<double>(nlij) * static_cast<double>(nlij);
+        }
         wave_sync();
-        float w = chol_solve<LMAX>(row, gdiag, 1.0f, n, lane, lmat, ldl);
+        float w = chol_solve<LMAX, PPW>(row, ginv, 1.0f, n, li, half, lmat);
         // FP64 residual r = R~ w - 1, up to two refinement steps
         for (int it = 0; it < 3; it++) {
             double r = 0.0;
 #pragma unroll
-            for (int k = 0; k < LMAX; k++)
-                if (k < n) r += static_cast<double>(orig[k]) * static_cast<double>(bcast(w, k));
-            r = (lane < n) ? r - 1.0 : 0.0;
-            const double rn = wave_sum(r * r);
-            if (!(rn > 1e-10 * n) || it == 2 || !ok) break;
-            w -= chol_solve<LMAX>(row, gdiag, static_cast<float>(r), n, lane, lmat, ldl);
+            for (int k = 0; k < LMAX; k++) {
+                const double wk = static_cast<double>(sub_bcast<PPW>(w, k, half));
+                if (k < n) r += static_cast<double>(orig[k]) * wk;
+            }
+            r = (li < n) ? r - 1.0 : 0.0;
+            const double rn = sub_sum<PPW>(r * r);
+            // refine only where single precision left a relative residual
+            // above 1e-3 (ill-conditioned systems); both halves of a wave
+            // take the same number of steps
+            const bool more = (rn > 1e-6 * n) && ok;
+            const bool any_more = __any(more);
+            if (!any_more || it == 2) break;
+            const float dw = chol_solve<LMAX, PPW>(row, ginv, static_cast<float>(r), n, li, half, lmat);
+            if (more) w -= dw;
         }
         // weights -> sample (applyWeights, RxBeamformerMV.cu:342-376)
         float xsum = 0.0f;
-        if (lane < n)
-            for (int k = 0; k < m.kact; k++) xsum += xs[ht * rs + m.first + k + lane];
-        const float wsum = wave_sum(lane < n ? w : 0.0f);
+        if (li < n)
+            for (int k = 0; k < m.kact; k++) xsum += xs[ht * rs + m.first + k + li];
+        const float wsum = sub_sum<PPW>(li < n ? w : 0.0f);
         const float scaling = 1.0f / (wsum * powf(static_cast<float>(m.kact), a.p));
-        float y = wave_sum(lane < n ? xsum * w * scaling : 0.0f);
-        if (fabsf(y) > 1e7f || isnan(y)) y = 0.0f;
-        if (lane == 0) out[static_cast<uint64_t>(t) * a.n_rx + s] = clamp_out<OutT>(y * static_cast<float>(a.n_ch));
+        float y = sub_sum<PPW>(li < n ? xsum * w * scaling : 0.0f);
+        if (fabsf(y) > 1e7f || isnan(y) || bad) y = 0.0f;
+        if (li == 0 && valid) out[static_cast<uint64_t>(t) * a.n_rx + s] = clamp_out<OutT>(y * static_cast<float>(a.n_ch));
         wave_sync();
     }
 }
@@ -478,8 +559,9 @@ size_t plan_lds(MvArgs& a, uint32_t tile) {
     off += static_cast<size_t>(H) * a.packed * 4;
     off = (off + 15) & ~static_cast<size_t>(15);
     a.off_wave = static_cast<uint32_t>(off);
-    a.wave_bytes = static_cast<uint32_t>(((a.L * (a.L + 1) * 4) + 15) & ~15u);
-    off += static_cast<size_t>(a.waves) * a.wave_bytes;
+    a.wave_bytes = static_cast<uint32_t>(((a.L * (a.L + 1) / 2 * 4) + 15) & ~15u);
+    const uint32_t ppw = a.L <= 32 ? 2 : 1;   // pixels per wave in phase 3
+    off += static_cast<size_t>(a.waves) * ppw * a.wave_bytes;
     a.off_fused = static_cast<uint32_t>(off);
     if (a.g.rf) off += (kWindowEntries + 4 * static_cast<size_t>(a.n_ch) + 6 * static_cast<size_t>(H)) * 4;
     return off;
@@ -511,13 +593,25 @@ static int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, 
     a.L = L;
     a.T = temporal_smoothing;
     a.K = num_channels - L + 1;
-    a.rs = num_channels + L + 4;
+    a.rs = num_channels + L + 8;   // zero padding for the sliding windows of the covariance phase
     a.packed = L * (L + 1) / 2;
     a.p = static_cast<float>(sub_array_scaling_power);
     a.alpha = static_cast<float>(loading_factor);
     a.means = compute_means ? 1 : 0;
     const size_t budget = static_cast<size_t>(max_lds_per_block());
+    // Prefer two workgroups per CU (the solve phase is latency-bound; a
+    // second resident workgroup hides it) as long as the tile keeps the
+    // temporal halo overhead moderate; else one workgroup with the largest tile.
     uint32_t tile = 0;
+    const uint32_t min_tile_half = std::max<uint32_t>(8, 2 * a.T + 2);
+    a.waves = kWaves;
+    for (uint32_t dt = kMaxTile; dt >= min_tile_half; dt--) {
+        a.tile = dt;
+        if (plan_lds(a, dt) <= budget / 2) {
+            tile = dt;
+            break;
+        }
+    }
     for (uint32_t waves = kWaves; waves >= 1 && tile == 0; waves /= 2) {
         a.waves = waves;
         for (uint32_t dt = kMaxTile; dt >= 1; dt--) {
