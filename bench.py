@@ -245,6 +245,82 @@ def bench_mv(args, torch):
     }
 
 
+def bench_stream(args, torch, num: int = 48):
+    """Uneven batched stream: 48 seeded acquisitions (scanlines 64-256, depth
+    samples 1024-4096, apertures 32-128 channels, fs 20/40 MHz), each by
+    delay-and-sum or pre-delay + minimum variance (L = aperture/2, T = 5,
+    loading trace/(100 L)), all in one graph launch of persistent kernels
+    with autotuned tiles.  Latency = an acquisition's completion time after
+    the launch starts (device timestamps).  Compared with the same
+    acquisitions issued one call each on one stream."""
+    from supra_amd import workloads as W
+    from supra_amd import _native as N
+    from supra_amd.batch import DAS, MV, Batch
+    from supra_amd.beamformer import RxBeamformer, USRawData, mv_das_perform_rx_beamforming
+
+    acqs = W.uneven_stream(num, seed=5)
+    log(f"[bench] synthesizing the {num}-acquisition uneven stream on the GPU")
+    items, b_das, f_mv, pixels = [], 0, 0.0, 0
+    stream = torch.cuda.current_stream().cuda_stream
+    for w, mode in acqs:
+        g = w.geometry()
+        rf = torch.empty(w.rf_shape, dtype=torch.int16, device="cuda")
+        w.rf_device(rf.data_ptr(), stream=stream)
+        out = torch.empty((g.num_depths, g.num_rx), dtype=torch.float32, device="cuda")
+        it = dict(engine=RxBeamformer(g), rf=rf, out=out, fs_hz=w.fs_hz, mode=DAS if mode == "das" else MV,
+                  window_type=N.WINDOW_HANN, sub_array_size=w.num_channels // 2, temporal_smoothing=5,
+                  loading_factor=0.01, sub_array_scaling_power=1.5)
+        items.append(it)
+        pixels += g.num_rx * g.num_depths
+        if mode == "das":
+            b_das += das_bytes(w)
+        else:
+            f_mv += mv_flops_per_pixel(w.num_channels, w.num_channels // 2, 5) * g.num_rx * g.num_depths
+    torch.cuda.synchronize()
+    log("[bench] planning the batch (autotuner) and capturing the graph")
+    batch = Batch(items, autotune=True)
+
+    def step():
+        batch.launch()
+
+    steps = max(3, min(args.steps, 5))
+    wall_ms, ev_ms = time_launches(step, steps, 2, torch)
+    lat = []
+    for _ in range(steps):
+        batch.launch()
+        l, _ = batch.latencies()
+        lat.append(l)
+    lat = np.concatenate(lat) / 1e3
+
+    def sequential():
+        for it in items:
+            raw = USRawData(it["rf"], it["fs_hz"], it["engine"].geometry)
+            if it["mode"] == DAS:
+                it["engine"].perform_rx_beamforming(N.BF_DELAY_AND_SUM, raw, 1.0, 1540e3, N.WINDOW_HANN, 0.0, False,
+                                                    0, out=it["out"])
+            else:
+                mv_das_perform_rx_beamforming(it["engine"], raw, 1.0, 1540e3, N.WINDOW_HANN, 0.0,
+                                              it["sub_array_size"], 5, 1.5, False, 0.01, out=it["out"])
+
+    log("[bench] timing the same acquisitions one call each")
+    _, seq_ms = time_launches(sequential, 2, 1, torch)
+    config = batch.config()
+    batch.close()
+    return {
+        "workload": f"{num} acquisitions (seed 5): {sum(m == 'das' for _, m in acqs)} DAS, "
+                    f"{sum(m == 'mv' for _, m in acqs)} MV, {pixels} output samples",
+        "frames_per_s": num / (ev_ms * 1e-3),
+        "ms_per_launch": ev_ms,
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "sequential_calls_ms": seq_ms,
+        "speedup_vs_sequential_calls": seq_ms / ev_ms,
+        "algorithmic_das_bytes": b_das,
+        "algorithmic_mv_flop": f_mv,
+        "autotune": config,
+    }
+
+
 def cpu_baseline_3d(res, budget_s: float):
     """Host FP32 oracle on whole scanlines of the same volume, doubling the
     sample until it takes >= budget_s (or covers the volume)."""
@@ -285,6 +361,7 @@ def main() -> int:
     ap.add_argument("--no-2d", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-mv", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) so that "
                          "FETCH_SIZE/WRITE_SIZE counters of a profile can be scaled")
@@ -323,6 +400,8 @@ def main() -> int:
         extra["diff_2d"] = bench_diff(args, torch)
     if not args.no_mv:
         extra["mv_2d"] = bench_mv(args, torch)
+    if not args.no_stream:
+        extra["uneven_stream"] = bench_stream(args, torch)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_3d(res, args.cpu_budget)

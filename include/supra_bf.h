@@ -264,6 +264,54 @@ int supra_mv_das_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype,
                      double sub_array_scaling_power, int32_t compute_means, double loading_factor,
                      void* out, int32_t out_dtype, void* stream);
 
+/* ---- batched stream of uneven acquisitions ------------------------------
+ * No reference counterpart (the reference runs one node call per frame).
+ * A batch is a fixed list of acquisitions, each with its own context
+ * (geometry), RF and output buffers, beamformed by delay-and-sum (the
+ * arguments of supra_das_run) or by pre-delay + minimum variance (those of
+ * supra_mv_das_run).  supra_batch_create validates and plans the batch once
+ * (optionally timing tile candidates on the GPU: autotune_tiles != 0),
+ * captures the launch in a HIP graph, and keeps the pointers: later launches
+ * beamform whatever the RF buffers then hold.  supra_batch_launch replays
+ * the graph asynchronously on `stream`; supra_batch_latencies synchronises
+ * with the last launch and returns each acquisition's completion time after
+ * the launch started (device wall clock, microseconds) and the largest one.
+ * DAS items: 2D, RF int16/float32, image int16/float32.  MV items: 2D, RF
+ * int16/float32, float32 image.  At most 256 acquisitions per kernel class.
+ * Launches of one batch must not overlap (they share its queue counters);
+ * the contexts must outlive the batch. */
+#define SUPRA_BATCH_DAS 0
+#define SUPRA_BATCH_MV  1
+
+typedef struct supra_batch_item {
+    supra_bf_ctx* ctx;
+    const void* rf;
+    int32_t rf_dtype;
+    int32_t mode;                       /* SUPRA_BATCH_DAS or SUPRA_BATCH_MV */
+    uint32_t num_tx, num_channels, num_samples;
+    int32_t window_type;
+    double sampling_frequency_hz, f_number, speed_of_sound_mm_per_s;
+    float window_parameter;
+    int32_t interpolate_transmits;      /* DAS */
+    int32_t additional_offset;          /* DAS */
+    uint32_t sub_array_size;            /* MV (0: num_channels / 2) */
+    uint32_t temporal_smoothing;        /* MV */
+    int32_t compute_means;              /* MV */
+    double sub_array_scaling_power;     /* MV */
+    double loading_factor;              /* MV */
+    void* out;                          /* [num_depths][num_rx_scanlines] */
+    int32_t out_dtype;
+    int32_t reserved;
+} supra_batch_item;
+
+typedef struct supra_batch supra_batch;
+int  supra_batch_create(const supra_batch_item* items, uint32_t num_items, int32_t autotune_tiles,
+                        supra_batch** out_batch);
+int  supra_batch_launch(supra_batch* batch, void* stream);
+int  supra_batch_latencies(supra_batch* batch, double* frame_us, double* total_us);
+int  supra_batch_config(const supra_batch* batch, char* buffer, size_t buffer_size);
+void supra_batch_destroy(supra_batch* batch);
+
 /* Window LUT of the reference (WindowFunction.cpp:18-47, WindowFunction.h:122-138),
  * computed on the host: SUPRA_WINDOW_NUM_ENTRIES floats, max-normalised. */
 int supra_window_lut(int32_t window_type, float window_parameter, float* lut_out);

@@ -144,6 +144,11 @@ _SIGNATURES = {
                                    C.c_double, C.c_double, C.c_double, C.c_int32, C.c_float, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_int32, C.c_double, C.c_void_p, C.c_int32,
                                    C.c_void_p]),
+    "supra_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "supra_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "supra_batch_latencies": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "supra_batch_config": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "supra_batch_destroy": (None, [C.c_void_p]),
     "supra_window_lut": (C.c_int, [C.c_int32, C.c_float, C.POINTER(C.c_float)]),
     "supra_last_error": (C.c_char_p, []),
     "supra_version": (C.c_char_p, []),

@@ -24,7 +24,7 @@ LIB = PKG / "libsupra_bf.so"
 ARCH = os.environ.get("SUPRA_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["geometry.cpp", "bf_host.cpp", "status.cpp"]
-HIP_SOURCES = ["bf_context.hip", "das_kernels.hip", "das_adjoint.hip", "mv_kernels.hip", "synth.hip"]
+HIP_SOURCES = ["bf_context.hip", "das_kernels.hip", "das_adjoint.hip", "mv_kernels.hip", "synth.hip", "batch.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_FLAGS = [

@@ -116,6 +116,32 @@ struct AdjointArgs {
     float lut[kWindowEntries];
 };
 
+// ---- batched persistent launches ---------------------------------------------
+
+// Shared by the work-queue kernels of one batch launch: the next work item of
+// each queue, completed items per frame and each frame's completion time
+// (device wall clock), all cleared by the batch's reset kernel.
+struct BatchSync {
+    uint32_t* head;             // [1] next item of this queue
+    uint32_t* done;             // [batch frame] items completed
+    uint64_t* stamp;            // [batch frame] completion time
+};
+
+struct DasBatchFrame {
+    DasArgs a;                  // one frame (n_frames = 1)
+    int32_t rf_dtype, out_dtype, interp, map;
+    uint32_t n_items;           // work items (scanline x depth tile) of the frame
+    uint32_t frame;             // index in the batch
+    uint32_t pad[2];
+};
+
+struct DasBatchArgs {
+    const DasBatchFrame* frames;
+    const uint32_t* items;      // [item] = queue frame << 24 | item of the frame
+    uint32_t n_items;
+    BatchSync sync;
+};
+
 // Host helpers (bf_host.cpp).
 void build_window_lut(int window_type, float window_parameter, float* lut);
 bool scanline_selection(const supra_scanline_rx& s, int32_t* sel, int32_t* active_mask, float txw[4]);

@@ -496,6 +496,57 @@ __global__ __launch_bounds__(kThreads) void rawdelay2d_kernel(const DasArgs a) {
         for (uint32_t j = written; j < a.n_ch; j++) out[static_cast<uint64_t>(j) * a.n_depths + t.r] = clamp_cast<OutT>(0.0f);
 }
 
+// ---- persistent work queue over many 2D frames -----------------------------
+
+template <typename RFT, typename OutT>
+__device__ inline void das2d_batch_tile(const DasBatchFrame& fr, const DasArgs& a, const Tile& t, Lds2D& L) {
+    if (fr.interp) {
+        if (fr.map) das2d_tile<RFT, OutT, false, true, true>(a, fr.a.lut, t, L);
+        else das2d_tile<RFT, OutT, false, true, false>(a, fr.a.lut, t, L);
+    } else {
+        if (fr.map) das2d_tile<RFT, OutT, false, false, true>(a, fr.a.lut, t, L);
+        else das2d_tile<RFT, OutT, false, false, false>(a, fr.a.lut, t, L);
+    }
+}
+
+// Each workgroup takes the next (frame, scanline, depth tile) item until the
+// queue is empty; the last item of a frame records the frame's completion
+// time.  Every workgroup leaves the loop once the queue index passes n_items.
+__global__ __launch_bounds__(kThreads) void das2d_batch_kernel(const DasBatchArgs b) {
+    __shared__ Lds2D lds;
+    __shared__ uint32_t s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(b.sync.head, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= b.n_items) break;
+        const uint32_t code = b.items[item];
+        const DasBatchFrame& fr = b.frames[code >> 24];
+        const DasArgs a = fr.a;     // scalars of the frame (the LUT is read from fr)
+        const uint32_t local = code & 0xffffffu;
+        Tile t;
+        const uint32_t tile = local % a.depth_tiles;
+        t.s = local / a.depth_tiles;
+        t.frame = 0;
+        t.r0 = tile * kThreads;
+        t.r = t.r0 + threadIdx.x;
+        t.valid = t.r < a.n_depths;
+        if (fr.rf_dtype == SUPRA_INT16) {
+            if (fr.out_dtype == SUPRA_FLOAT32) das2d_batch_tile<int16_t, float>(fr, a, t, lds);
+            else das2d_batch_tile<int16_t, int16_t>(fr, a, t, lds);
+        } else {
+            if (fr.out_dtype == SUPRA_FLOAT32) das2d_batch_tile<float, float>(fr, a, t, lds);
+            else das2d_batch_tile<float, int16_t>(fr, a, t, lds);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(&b.sync.done[fr.frame], 1u) + 1u == fr.n_items) b.sync.stamp[fr.frame] = wall_clock64();
+        }
+    }
+}
+
 }  // namespace
 
 template <typename RFT, typename OutT>
@@ -523,6 +574,11 @@ static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool i
 }
 
 uint32_t das_depth_tile() { return kThreads; }
+
+hipError_t launch_das_batch(const DasBatchArgs& b, uint32_t workgroups, hipStream_t stream) {
+    hipLaunchKernelGGL(das2d_batch_kernel, dim3(workgroups), dim3(kThreads), 0, stream, b);
+    return hipGetLastError();
+}
 
 hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, hipStream_t stream) {
     const uint64_t blocks = static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames;

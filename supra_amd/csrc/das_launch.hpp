@@ -17,6 +17,9 @@ hipError_t launch_das(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, 
 // Launch the 2D pre-delay kernel (output [frame][scanline][channel][depth]).
 hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, hipStream_t stream);
 
+// Persistent work-queue DAS over the frames of a batch (das_kernels.hip).
+hipError_t launch_das_batch(const DasBatchArgs& b, uint32_t workgroups, hipStream_t stream);
+
 // Differentiable 2D DAS (das_adjoint.hip).
 hipError_t launch_frame_tables(const FrameTablesArgs& a, hipStream_t stream);
 hipError_t launch_beta(const float* grad_image, const float* norm, float* beta, uint32_t n_rx, uint32_t n_d,

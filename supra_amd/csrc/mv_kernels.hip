@@ -47,21 +47,6 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxTile = 64;
 
-struct MvArgs {
-    const void* x;
-    void* out;
-    uint32_t n_rx, n_ch, n_d, L, T, K;
-    uint32_t tile;          // output depths per workgroup (Dt)
-    uint32_t n_tiles;
-    uint32_t rs;            // LDS row stride of x (channels + padding)
-    uint32_t packed;        // L (L + 1) / 2
-    uint32_t waves;         // waves that solve (phase 3)
-    uint32_t off_meta, off_mean, off_r, off_wave, wave_bytes;   // LDS byte offsets
-    uint32_t off_fused;     // LDS byte offset of the pre-delay tables (fused mode)
-    float p, alpha;
-    int means;
-    DasArgs g;              // fused mode: geometry and RF of the pre-delay
-};
 
 template <typename T> __device__ inline float in_f(T v) { return static_cast<float>(v); }
 
@@ -532,6 +517,38 @@ hipError_t launch_io(const MvArgs& a, int in_dtype, int out_dtype, size_t lds, h
                                         : launch_lmax<float, int16_t, FUSED>(a, lds, hs);
 }
 
+// ---- persistent work queue over many frames (fused pre-delay + MV) --------
+
+template <int LMAX>
+__global__ __launch_bounds__(kThreads) void mv_batch_kernel(const MvBatchArgs b) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(b.sync.head, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= b.n_items) break;
+        const uint32_t code = b.items[item];
+        const MvBatchFrame& fr = b.frames[code >> 24];
+        const MvArgs a = fr.a;     // scalars of the frame (the LUT is read from fr)
+        const uint32_t local = code & 0xffffffu;
+        const uint32_t s = local / a.n_tiles, tile = local % a.n_tiles;
+        if (fr.rf_dtype == SUPRA_INT16) {
+            if (a.means) mv_tile<int16_t, float, LMAX, true, true>(a, fr.a.g.lut, s, tile, lds);
+            else mv_tile<int16_t, float, LMAX, false, true>(a, fr.a.g.lut, s, tile, lds);
+        } else {
+            if (a.means) mv_tile<float, float, LMAX, true, true>(a, fr.a.g.lut, s, tile, lds);
+            else mv_tile<float, float, LMAX, false, true>(a, fr.a.g.lut, s, tile, lds);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(&b.sync.done[fr.frame], 1u) + 1u == fr.n_items) b.sync.stamp[fr.frame] = wall_clock64();
+        }
+    }
+}
+
 int max_lds_per_block() {
     static std::mutex mu;
     static int cached[64] = {0};
@@ -575,9 +592,9 @@ using namespace supra_amd;
 namespace supra_amd {
 
 // Argument checks and the LDS/tile plan shared by both entry points.
-static int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, uint32_t num_samples,
-                      uint32_t sub_array_size, uint32_t temporal_smoothing, double sub_array_scaling_power,
-                      int32_t compute_means, double loading_factor, size_t* lds_out) {
+int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, uint32_t num_samples,
+               uint32_t sub_array_size, uint32_t temporal_smoothing, double sub_array_scaling_power,
+               int32_t compute_means, double loading_factor, size_t lds_budget, size_t* lds_out) {
     if (num_scanlines == 0 || num_channels == 0 || num_samples == 0)
         return fail(SUPRA_ERR_INVALID_ARG, "mv: empty input");
     const uint32_t L = sub_array_size == 0 ? num_channels / 2 : sub_array_size;
@@ -598,12 +615,14 @@ static int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, 
     a.p = static_cast<float>(sub_array_scaling_power);
     a.alpha = static_cast<float>(loading_factor);
     a.means = compute_means ? 1 : 0;
-    const size_t budget = static_cast<size_t>(max_lds_per_block());
-    // Prefer two workgroups per CU (the solve phase is latency-bound; a
-    // second resident workgroup hides it) as long as the tile keeps the
-    // temporal halo overhead moderate; else one workgroup with the largest tile.
+    const size_t budget = lds_budget ? std::min(lds_budget, static_cast<size_t>(max_lds_per_block()))
+                                     : static_cast<size_t>(max_lds_per_block());
+    // Without an explicit budget, prefer two workgroups per CU (the solve
+    // phase is latency-bound; a second resident workgroup hides it) as long as
+    // the tile keeps the temporal halo overhead moderate; else one workgroup
+    // with the largest tile.
     uint32_t tile = 0;
-    const uint32_t min_tile_half = std::max<uint32_t>(8, 2 * a.T + 2);
+    const uint32_t min_tile_half = lds_budget ? kMaxTile + 1 : std::max<uint32_t>(8, 2 * a.T + 2);
     a.waves = kWaves;
     for (uint32_t dt = kMaxTile; dt >= min_tile_half; dt--) {
         a.tile = dt;
@@ -631,6 +650,12 @@ static int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, 
     return SUPRA_OK;
 }
 
+hipError_t launch_mv_batch(const MvBatchArgs& b, uint32_t lmax, uint32_t workgroups, size_t lds, hipStream_t stream) {
+    if (lmax <= 32) hipLaunchKernelGGL(mv_batch_kernel<32>, dim3(workgroups), dim3(kThreads), lds, stream, b);
+    else hipLaunchKernelGGL(mv_batch_kernel<64>, dim3(workgroups), dim3(kThreads), lds, stream, b);
+    return hipGetLastError();
+}
+
 int launch_mv_fused(const DasArgs& g, int32_t rf_dtype, uint32_t sub_array_size, uint32_t temporal_smoothing,
                     double sub_array_scaling_power, int32_t compute_means, double loading_factor, void* out,
                     int32_t out_dtype, hipStream_t stream) {
@@ -640,7 +665,7 @@ int launch_mv_fused(const DasArgs& g, int32_t rf_dtype, uint32_t sub_array_size,
     a.out = out;
     size_t lds = 0;
     int st = mv_prepare(a, g.n_rx, g.n_ch, g.n_depths, sub_array_size, temporal_smoothing, sub_array_scaling_power,
-                        compute_means, loading_factor, &lds);
+                        compute_means, loading_factor, 0, &lds);
     if (st != SUPRA_OK) return st;
     hipError_t e = launch_io<true>(a, rf_dtype, out_dtype, lds, stream);
     if (e != hipSuccess) return fail(SUPRA_ERR_HIP, "mv: launch: %s", hipGetErrorString(e));
@@ -664,7 +689,7 @@ extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t 
     a.out = out;
     size_t lds = 0;
     int st = mv_prepare(a, num_scanlines, num_channels, num_samples, sub_array_size, temporal_smoothing,
-                        sub_array_scaling_power, compute_means, loading_factor, &lds);
+                        sub_array_scaling_power, compute_means, loading_factor, 0, &lds);
     if (st != SUPRA_OK) return st;
     hipError_t e = launch_io<false>(a, in_dtype, out_dtype, lds, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(SUPRA_ERR_HIP, "mv: launch: %s", hipGetErrorString(e));

@@ -176,4 +176,46 @@ def matrix_3d_small(n_el: int = 8, n_lines: int = 8, n_depths: int = 200, n_samp
     return replace(w, **overrides) if overrides else w
 
 
+def linear_acquisition(num_rx: int, num_depths: int, aperture: int, fs_hz: float, num_scatterers: int,
+                       scatter_seed: int, noise_seed: int, **overrides) -> Workload:
+    """A linear-array acquisition of the uneven stream: num_rx elements at
+    0.3 mm pitch and as many focused transmits (one scanline each), a receive
+    and transmit aperture of `aperture` elements (at most the array) on
+    `aperture` channels, and num_depths RF samples imaged to the depth they
+    reach (c = 1540 m/s)."""
+    c_mm_per_s = 1540.0e3
+    elements = min(aperture, num_rx)
+    depth = num_depths / fs_hz * c_mm_per_s / 2.0
+    half = num_rx * 0.3 / 2.0
+    bf = BeamformerConfig(scan_type="linear", transducer_type="linear", element_layout=(num_rx, 1),
+                          pitch=(0.3, 0.3), num_scanlines=(num_rx, 1), max_aperture=(elements, 1),
+                          tx_max_aperture=(elements, 1), depth=depth, num_depths=num_depths, tx_focus_active=True,
+                          tx_focus_depth=depth / 2.0, speed_of_sound=1540.0)
+    w = Workload(name=f"linear_{num_rx}x{num_depths}_a{aperture}_{fs_hz / 1e6:.0f}MHz", bf=bf, fs_hz=fs_hz,
+                 num_channels=aperture, num_samples=num_depths, pulse_hz=fs_hz / 6.0,
+                 num_scatterers=num_scatterers, box_x=(-half, half), box_depth=(1.0, depth),
+                 scatter_seed=scatter_seed, noise_seed=noise_seed, beam_sigma_mm=1.0)
+    return replace(w, **overrides) if overrides else w
+
+
+def uneven_stream(num: int = 48, seed: int = 5):
+    """The uneven batched stream: `num` seeded acquisitions with scanlines in
+    {64, 128, 192, 256}, depth samples in [1024, 4096], aperture in
+    {32, 64, 128} channels, fs in {20, 40} MHz and 100-400 scatterers, each
+    beamformed by delay-and-sum or by minimum variance (L = aperture / 2,
+    T = 5).  Returns [(Workload, mode)] with mode "das" or "mv"."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(num):
+        n_rx = int(rng.choice([64, 128, 192, 256]))
+        n_d = int(rng.integers(1024, 4097))
+        aperture = int(rng.choice([32, 64, 128]))
+        fs = float(rng.choice([20e6, 40e6]))
+        n_sc = int(rng.integers(100, 401))
+        mode = "mv" if rng.random() < 0.5 else "das"
+        out.append((linear_acquisition(n_rx, n_d, aperture, fs, n_sc, scatter_seed=1000 + i,
+                                       noise_seed=2000 + i), mode))
+    return out
+
+
 WORKLOADS = {"linear_2d": linear_2d, "matrix_3d": matrix_3d}
