@@ -3,15 +3,17 @@
 // call, BeamformingNode.cpp:128-177, and one RawDelay + MV chain per frame).
 //
 // A batch is planned once: every acquisition becomes a frame of work items
-// (scanline x tile of output depths) in one of three queues, by the register
-// and LDS footprint of its kernel: delay-and-sum, minimum variance with
-// L <= 32 and with L <= 64 (the pre-delay fused in).  Each queue is served by
-// a persistent work-queue kernel sized to the GPU; workgroups take items in
-// frame order, so frames complete in submission order within a queue, and
-// the last item of a frame stamps the frame's completion time from the device
-// wall clock.  The queues run concurrently (a fork/join of three streams) and
-// the whole launch — reset of the queue counters, the three kernels — is
-// captured once in a hipGraph; supra_batch_launch replays it.
+// (scanline x tile of output depths) in a queue of its kernel class:
+// delay-and-sum (one queue, the RF/image types and options are per-frame
+// branches), or minimum variance with the pre-delay fused in, one queue per
+// (L <= 32 or L <= 64, RF type, mean removal) so that each runs a kernel
+// specialised for it.  Each queue is served by a persistent work-queue
+// kernel sized to the GPU; workgroups take items in frame order, so frames
+// complete in submission order within a queue, and the last item of a frame
+// stamps the frame's completion time from the device wall clock.  The queues
+// run concurrently (a fork/join over one stream per queue) and the whole
+// launch — reset of the queue counters, the kernels — is captured once in a
+// hipGraph; supra_batch_launch replays it.
 //
 // Footprints differ by an order of magnitude (DAS: 70 VGPRs, 10 KB LDS; MV:
 // 256 VGPRs and up to the whole LDS), so one kernel for all items would run
@@ -43,13 +45,13 @@ using namespace supra_amd;
 
 namespace {
 
-constexpr uint32_t kQueues = 3;            // DAS, MV (L <= 32), MV (L <= 64)
+constexpr uint32_t kMaxQueues = 9;         // DAS + MV x {L<=32, L<=64} x {int16, float} x {means}
 constexpr uint32_t kMaxFramesPerQueue = 256;
 constexpr uint32_t kMaxItemsPerFrame = 1u << 24;
 
 __global__ void batch_reset_kernel(uint32_t* heads, uint32_t* done, uint64_t* stamp, uint32_t n_frames) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kQueues) heads[i] = 0;
+    if (i < kMaxQueues) heads[i] = 0;
     if (i < n_frames) {
         done[i] = 0;
         stamp[i] = 0;
@@ -59,9 +61,23 @@ __global__ void batch_reset_kernel(uint32_t* heads, uint32_t* done, uint64_t* st
 
 struct FrameDesc {
     supra_batch_item item;
-    uint32_t queue;            // 0 DAS, 1 MV32, 2 MV64
+    uint32_t queue;            // index into supra_batch::queues
     double dt;
     size_t table_offset;       // bytes into the table block
+};
+
+struct Queue {
+    bool mv = false;
+    uint32_t lmax = 0;         // MV: 32 or 64
+    int32_t rf_dtype = 0;      // MV
+    int32_t means = 0;         // MV
+    void* frames = nullptr;    // DasBatchFrame[] or MvBatchFrame[] (device)
+    uint32_t* items = nullptr;
+    uint32_t n_items = 0, n_frames = 0;
+    size_t lds = 0, budget = 0;
+    uint32_t wg_per_cu = 1;
+    hipStream_t stream = nullptr;
+    hipEvent_t joined = nullptr;
 };
 
 }  // namespace
@@ -72,28 +88,17 @@ struct supra_batch {
     int cus = 0;
     double clock_khz = 0;
     std::vector<FrameDesc> frames;
+    std::vector<Queue> queues;
     // device memory
     void* tables = nullptr;            // per-frame dt-space tables
-    size_t tables_bytes = 0;
-    uint32_t* sync = nullptr;          // heads[kQueues] + done[n_frames] (uint32) + stamps (uint64)
+    uint32_t* sync = nullptr;          // heads[kMaxQueues] + done[n_frames] (uint32) + stamps (uint64)
     uint64_t* stamp = nullptr;
     uint32_t* heads = nullptr;
     uint32_t* done = nullptr;
-    DasBatchFrame* das_frames = nullptr;
-    uint32_t* das_items = nullptr;
-    uint32_t das_n_items = 0, das_n_frames = 0;
-    MvBatchFrame* mv_frames[2] = {nullptr, nullptr};
-    uint32_t* mv_items[2] = {nullptr, nullptr};
-    uint32_t mv_n_items[2] = {0, 0}, mv_n_frames[2] = {0, 0};
-    size_t mv_lds[2] = {0, 0};
-    // chosen configuration
-    uint32_t das_wg_per_cu = 4;
-    uint32_t mv_wg_per_cu[2] = {1, 1};
-    size_t mv_budget[2] = {0, 0};
     std::string config;
     // graph
-    hipStream_t streams[kQueues] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    hipStream_t origin = nullptr;      // capture stream: reset, fork, join
+    hipEvent_t ev_fork = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t last_stream = nullptr;
@@ -102,18 +107,15 @@ struct supra_batch {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
-        for (auto& e : ev_join)
-            if (e) (void)hipEventDestroy(e);
-        for (auto& s : streams)
-            if (s) (void)hipStreamDestroy(s);
+        if (origin) (void)hipStreamDestroy(origin);
+        for (Queue& q : queues) {
+            if (q.joined) (void)hipEventDestroy(q.joined);
+            if (q.stream) (void)hipStreamDestroy(q.stream);
+            (void)hipFree(q.frames);
+            (void)hipFree(q.items);
+        }
         (void)hipFree(tables);
         (void)hipFree(sync);
-        (void)hipFree(das_frames);
-        (void)hipFree(das_items);
-        for (int q = 0; q < 2; q++) {
-            (void)hipFree(mv_frames[q]);
-            (void)hipFree(mv_items[q]);
-        }
     }
 };
 
@@ -194,23 +196,29 @@ DasArgs frame_das_args(const supra_batch* b, const FrameDesc& fd) {
     return a;
 }
 
-template <typename T>
-int upload(T** dst, const std::vector<T>& src, const char* what) {
+template <typename T, typename D>
+int upload(D** dst, const std::vector<T>& src, const char* what) {
     (void)hipFree(*dst);
     *dst = nullptr;
     if (src.empty()) return SUPRA_OK;
-    hipError_t e = hipMalloc(dst, src.size() * sizeof(T));
-    if (e == hipSuccess) e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_fail(e, what);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, src.size() * sizeof(T));
+    if (e == hipSuccess) e = hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, what);
+    }
+    *dst = static_cast<D*>(p);
     return SUPRA_OK;
 }
 
-int build_das_queue(supra_batch* b) {
+int build_das_queue(supra_batch* b, uint32_t qi) {
+    Queue& q = b->queues[qi];
     std::vector<DasBatchFrame> frames;
     std::vector<uint32_t> items;
     for (uint32_t f = 0; f < b->n_frames; f++) {
         const FrameDesc& fd = b->frames[f];
-        if (fd.queue != 0) continue;
+        if (fd.queue != qi) continue;
         DasBatchFrame fr;
         std::memset(&fr, 0, sizeof(fr));
         fr.a = frame_das_args(b, fd);
@@ -219,26 +227,28 @@ int build_das_queue(supra_batch* b) {
         fr.interp = fd.item.interpolate_transmits ? 1 : 0;
         fr.map = fd.item.ctx->d_e2c ? 1 : 0;
         fr.n_items = fr.a.n_rx * fr.a.depth_tiles;
+        if (fr.n_items >= kMaxItemsPerFrame) return fail(SUPRA_ERR_TOO_LARGE, "batch: frame %u too large", f);
         fr.frame = f;
         const uint32_t qf = static_cast<uint32_t>(frames.size());
         for (uint32_t i = 0; i < fr.n_items; i++) items.push_back(qf << 24 | i);
         frames.push_back(fr);
     }
-    b->das_n_frames = static_cast<uint32_t>(frames.size());
-    b->das_n_items = static_cast<uint32_t>(items.size());
-    int st = upload(&b->das_frames, frames, "batch: DAS frames");
-    if (st == SUPRA_OK) st = upload(&b->das_items, items, "batch: DAS items");
+    q.n_frames = static_cast<uint32_t>(frames.size());
+    q.n_items = static_cast<uint32_t>(items.size());
+    int st = upload(&q.frames, frames, "batch: DAS frames");
+    if (st == SUPRA_OK) st = upload(&q.items, items, "batch: DAS items");
     return st;
 }
 
-// Plans the MV frames of queue q (1: L <= 32, 2: L <= 64) for an LDS budget.
-int build_mv_queue(supra_batch* b, uint32_t q, size_t budget) {
+// Plans the MV frames of queue qi for an LDS budget.
+int build_mv_queue(supra_batch* b, uint32_t qi, size_t budget) {
+    Queue& q = b->queues[qi];
     std::vector<MvBatchFrame> frames;
     std::vector<uint32_t> items;
     size_t lds = 0;
     for (uint32_t f = 0; f < b->n_frames; f++) {
         const FrameDesc& fd = b->frames[f];
-        if (fd.queue != q) continue;
+        if (fd.queue != qi) continue;
         const supra_batch_item& it = fd.item;
         MvBatchFrame fr;
         std::memset(&fr, 0, sizeof(fr));
@@ -258,11 +268,12 @@ int build_mv_queue(supra_batch* b, uint32_t q, size_t budget) {
         for (uint32_t i = 0; i < fr.n_items; i++) items.push_back(qf << 24 | i);
         frames.push_back(fr);
     }
-    b->mv_n_frames[q - 1] = static_cast<uint32_t>(frames.size());
-    b->mv_n_items[q - 1] = static_cast<uint32_t>(items.size());
-    b->mv_lds[q - 1] = lds;
-    int st = upload(&b->mv_frames[q - 1], frames, "batch: MV frames");
-    if (st == SUPRA_OK) st = upload(&b->mv_items[q - 1], items, "batch: MV items");
+    q.n_frames = static_cast<uint32_t>(frames.size());
+    q.n_items = static_cast<uint32_t>(items.size());
+    q.lds = lds;
+    q.budget = budget;
+    int st = upload(&q.frames, frames, "batch: MV frames");
+    if (st == SUPRA_OK) st = upload(&q.items, items, "batch: MV items");
     return st;
 }
 
@@ -284,37 +295,35 @@ BatchSync make_sync(const supra_batch* b, uint32_t q) {
 }
 
 hipError_t launch_reset(const supra_batch* b, hipStream_t s) {
-    const uint32_t n = std::max(b->n_frames + 1, kQueues);
+    const uint32_t n = std::max(b->n_frames + 1, kMaxQueues);
     hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, s, b->heads, b->done, b->stamp,
                        b->n_frames);
     return hipGetLastError();
 }
 
-hipError_t launch_queue(const supra_batch* b, uint32_t q, hipStream_t s) {
-    if (q == 0) {
-        if (b->das_n_items == 0) return hipSuccess;
+hipError_t launch_queue(const supra_batch* b, uint32_t qi, hipStream_t s) {
+    const Queue& q = b->queues[qi];
+    if (q.n_items == 0) return hipSuccess;
+    const uint32_t wg = std::min<uint32_t>(q.n_items, q.wg_per_cu * static_cast<uint32_t>(b->cus));
+    if (!q.mv) {
         DasBatchArgs a;
-        a.frames = b->das_frames;
-        a.items = b->das_items;
-        a.n_items = b->das_n_items;
-        a.sync = make_sync(b, 0);
-        const uint32_t wg = std::min<uint32_t>(b->das_n_items, b->das_wg_per_cu * static_cast<uint32_t>(b->cus));
+        a.frames = static_cast<const DasBatchFrame*>(q.frames);
+        a.items = q.items;
+        a.n_items = q.n_items;
+        a.sync = make_sync(b, qi);
         return launch_das_batch(a, wg, s);
     }
-    const uint32_t m = q - 1;
-    if (b->mv_n_items[m] == 0) return hipSuccess;
     MvBatchArgs a;
-    a.frames = b->mv_frames[m];
-    a.items = b->mv_items[m];
-    a.n_items = b->mv_n_items[m];
-    a.sync = make_sync(b, q);
-    const uint32_t wg = std::min<uint32_t>(b->mv_n_items[m], b->mv_wg_per_cu[m] * static_cast<uint32_t>(b->cus));
-    return launch_mv_batch(a, q == 1 ? 32 : 64, wg, b->mv_lds[m], s);
+    a.frames = static_cast<const MvBatchFrame*>(q.frames);
+    a.items = q.items;
+    a.n_items = q.n_items;
+    a.sync = make_sync(b, qi);
+    return launch_mv_batch(a, q.lmax, q.rf_dtype, q.means, wg, q.lds, s);
 }
 
 // Average time of one launch of queue q alone (after a reset), in ms.
 int time_queue(supra_batch* b, uint32_t q, float* ms) {
-    hipStream_t s = b->streams[0];
+    hipStream_t s = b->origin;
     hipEvent_t e0, e1;
     hipError_t e;
     if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(e, "batch: event");
@@ -341,38 +350,40 @@ int time_queue(supra_batch* b, uint32_t q, float* ms) {
 }
 
 int autotune(supra_batch* b) {
-    char line[160];
+    char line[200];
     int st;
-    if (b->das_n_items) {
-        const uint32_t cands[] = {2, 4, 7};
-        float best = 1e30f;
-        uint32_t pick = 4;
-        for (uint32_t c : cands) {
-            b->das_wg_per_cu = c;
-            float t;
-            if ((st = time_queue(b, 0, &t)) != SUPRA_OK) return st;
-            std::snprintf(line, sizeof(line), "das wg/cu=%u %.3f ms; ", c, t);
-            b->config += line;
-            if (t < best) {
-                best = t;
-                pick = c;
+    for (uint32_t qi = 0; qi < b->queues.size(); qi++) {
+        Queue& q = b->queues[qi];
+        if (!q.n_items) continue;
+        if (!q.mv) {
+            const uint32_t cands[] = {2, 4, 7};
+            float best = 1e30f;
+            uint32_t pick = 4;
+            for (uint32_t c : cands) {
+                q.wg_per_cu = c;
+                float t;
+                if ((st = time_queue(b, qi, &t)) != SUPRA_OK) return st;
+                std::snprintf(line, sizeof(line), "das wg/cu=%u %.3f ms; ", c, t);
+                b->config += line;
+                if (t < best) {
+                    best = t;
+                    pick = c;
+                }
             }
+            q.wg_per_cu = pick;
+            continue;
         }
-        b->das_wg_per_cu = pick;
-    }
-    for (uint32_t q = 1; q <= 2; q++) {
-        if (!b->mv_n_items[q - 1]) continue;
         struct Cand { size_t budget; uint32_t wg; };
         const Cand cands[] = {{mv_full_budget(b), 1}, {mv_half_budget(b), 2}};
         float best = 1e30f;
         Cand pick = cands[0];
         for (const Cand& c : cands) {
-            if ((st = build_mv_queue(b, q, c.budget)) != SUPRA_OK) continue;   // does not fit: skip
-            b->mv_wg_per_cu[q - 1] = c.wg;
+            if (build_mv_queue(b, qi, c.budget) != SUPRA_OK) continue;   // does not fit: skip
+            q.wg_per_cu = c.wg;
             float t;
-            if ((st = time_queue(b, q, &t)) != SUPRA_OK) return st;
-            std::snprintf(line, sizeof(line), "mv%s lds=%zu wg/cu=%u %.3f ms; ", q == 1 ? "32" : "64", c.budget,
-                          c.wg, t);
+            if ((st = time_queue(b, qi, &t)) != SUPRA_OK) return st;
+            std::snprintf(line, sizeof(line), "mv L<=%u %s%s lds=%zu wg/cu=%u %.3f ms; ", q.lmax,
+                          q.rf_dtype == SUPRA_INT16 ? "int16" : "float", q.means ? " means" : "", c.budget, c.wg, t);
             b->config += line;
             if (t < best) {
                 best = t;
@@ -380,24 +391,25 @@ int autotune(supra_batch* b) {
             }
         }
         clear_error();
-        b->mv_budget[q - 1] = pick.budget;
-        b->mv_wg_per_cu[q - 1] = pick.wg;
-        if ((st = build_mv_queue(b, q, pick.budget)) != SUPRA_OK) return st;
+        q.wg_per_cu = pick.wg;
+        if ((st = build_mv_queue(b, qi, pick.budget)) != SUPRA_OK) return st;
     }
     return SUPRA_OK;
 }
 
 int capture_graph(supra_batch* b) {
     hipError_t e;
-    hipStream_t s0 = b->streams[0];
+    hipStream_t s0 = b->origin;
     if ((e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return hip_fail(e, "batch: begin capture");
     e = launch_reset(b, s0);
     if (e == hipSuccess) e = hipEventRecord(b->ev_fork, s0);
-    for (uint32_t q = 1; q < kQueues && e == hipSuccess; q++) e = hipStreamWaitEvent(b->streams[q], b->ev_fork, 0);
-    for (uint32_t q = 0; q < kQueues && e == hipSuccess; q++) e = launch_queue(b, q, b->streams[q]);
-    for (uint32_t q = 1; q < kQueues && e == hipSuccess; q++) e = hipEventRecord(b->ev_join[q - 1], b->streams[q]);
-    for (uint32_t q = 1; q < kQueues && e == hipSuccess; q++) e = hipStreamWaitEvent(s0, b->ev_join[q - 1], 0);
+    for (uint32_t q = 0; q < b->queues.size() && e == hipSuccess; q++) {
+        e = hipStreamWaitEvent(b->queues[q].stream, b->ev_fork, 0);
+        if (e == hipSuccess) e = launch_queue(b, q, b->queues[q].stream);
+        if (e == hipSuccess) e = hipEventRecord(b->queues[q].joined, b->queues[q].stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->queues[q].joined, 0);
+    }
     hipGraph_t g = nullptr;
     const hipError_t e_end = hipStreamEndCapture(s0, &g);
     if (e != hipSuccess) {
@@ -437,21 +449,33 @@ extern "C" int supra_batch_create(const supra_batch_item* items, uint32_t num_it
     b->clock_khz = khz > 0 ? khz : 100000.0;
     if (b->cus <= 0) b->cus = 1;
     b->n_frames = num_items;
-    uint32_t per_queue[kQueues] = {0, 0, 0};
+    std::vector<uint32_t> per_queue;
     size_t offset = 0;
     for (uint32_t i = 0; i < num_items; i++) {
         FrameDesc fd;
         fd.item = items[i];
         if (items[i].ctx->device != b->device)
             return fail(SUPRA_ERR_INVALID_ARG, "batch item %u: context of another device", i);
-        if (items[i].mode == SUPRA_BATCH_DAS) {
-            fd.queue = 0;
-        } else {
+        Queue key;
+        if (items[i].mode == SUPRA_BATCH_MV) {
             const uint32_t L = items[i].sub_array_size ? items[i].sub_array_size : items[i].num_channels / 2;
-            fd.queue = L <= 32 ? 1 : 2;
+            key.mv = true;
+            key.lmax = L <= 32 ? 32 : 64;
+            key.rf_dtype = items[i].rf_dtype;
+            key.means = items[i].compute_means ? 1 : 0;
         }
-        if (++per_queue[fd.queue] > kMaxFramesPerQueue)
+        uint32_t qi = 0;
+        while (qi < b->queues.size() && !(b->queues[qi].mv == key.mv && b->queues[qi].lmax == key.lmax &&
+                                          b->queues[qi].rf_dtype == key.rf_dtype && b->queues[qi].means == key.means))
+            qi++;
+        if (qi == b->queues.size()) {
+            if (qi == kMaxQueues) return fail(SUPRA_ERR_TOO_LARGE, "batch: too many kernel classes");
+            b->queues.push_back(key);
+            per_queue.push_back(0);
+        }
+        if (++per_queue[qi] > kMaxFramesPerQueue)
             return fail(SUPRA_ERR_TOO_LARGE, "batch: at most %u frames per kernel class", kMaxFramesPerQueue);
+        fd.queue = qi;
         fd.dt = 1.0 / items[i].sampling_frequency_hz;
         fd.table_offset = offset;
         offset += table_bytes(items[i].ctx);
@@ -470,39 +494,48 @@ extern "C" int supra_batch_create(const supra_batch_item* items, uint32_t num_it
         base += align256(ctx->n_depths * sizeof(float));
         std::memcpy(base, ctx->h_ex.data(), ctx->n_el * sizeof(float));
     }
-    b->tables_bytes = offset;
     if ((e = hipMalloc(&b->tables, offset)) != hipSuccess ||
         (e = hipMemcpy(b->tables, host.data(), offset, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "batch: tables");
-    const size_t sync_bytes = align256((kQueues + num_items) * sizeof(uint32_t)) + (num_items + 1) * sizeof(uint64_t);
+    const size_t sync_bytes =
+        align256((kMaxQueues + num_items) * sizeof(uint32_t)) + (num_items + 1) * sizeof(uint64_t);
     if ((e = hipMalloc(&b->sync, sync_bytes)) != hipSuccess) return hip_fail(e, "batch: counters");
     b->heads = b->sync;
-    b->done = b->sync + kQueues;
+    b->done = b->sync + kMaxQueues;
     b->stamp = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(b->sync) +
-                                           align256((kQueues + num_items) * sizeof(uint32_t)));
-    for (auto& s : b->streams)
-        if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "batch: stream");
-    if ((e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&b->ev_join[0], hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&b->ev_join[1], hipEventDisableTiming)) != hipSuccess)
-        return hip_fail(e, "batch: event");
-    int st = build_das_queue(b);
-    for (uint32_t q = 1; q <= 2 && st == SUPRA_OK; q++) {
-        st = build_mv_queue(b, q, mv_full_budget(b));
-        b->mv_budget[q - 1] = mv_full_budget(b);
-        b->mv_wg_per_cu[q - 1] = 1;
+                                           align256((kMaxQueues + num_items) * sizeof(uint32_t)));
+    if ((e = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "batch: stream");
+    for (Queue& q : b->queues)
+        if ((e = hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&q.joined, hipEventDisableTiming)) != hipSuccess)
+            return hip_fail(e, "batch: stream");
+    int st = SUPRA_OK;
+    for (uint32_t qi = 0; qi < b->queues.size() && st == SUPRA_OK; qi++) {
+        Queue& q = b->queues[qi];
+        if (!q.mv) {
+            q.wg_per_cu = 4;
+            st = build_das_queue(b, qi);
+        } else {
+            q.wg_per_cu = 1;
+            st = build_mv_queue(b, qi, mv_full_budget(b));
+        }
     }
     if (st != SUPRA_OK) return st;
     b->config.clear();
     if (autotune_tiles && (st = autotune(b)) != SUPRA_OK) return st;
-    char line[200];
-    std::snprintf(line, sizeof(line),
-                  "chosen: das frames=%u items=%u wg/cu=%u | mv32 frames=%u items=%u wg/cu=%u lds=%zu | "
-                  "mv64 frames=%u items=%u wg/cu=%u lds=%zu",
-                  b->das_n_frames, b->das_n_items, b->das_wg_per_cu, b->mv_n_frames[0], b->mv_n_items[0],
-                  b->mv_wg_per_cu[0], b->mv_lds[0], b->mv_n_frames[1], b->mv_n_items[1], b->mv_wg_per_cu[1],
-                  b->mv_lds[1]);
-    b->config += line;
+    b->config += "chosen:";
+    for (const Queue& q : b->queues) {
+        char line[200];
+        if (!q.mv)
+            std::snprintf(line, sizeof(line), " [das frames=%u items=%u wg/cu=%u]", q.n_frames, q.n_items, q.wg_per_cu);
+        else
+            std::snprintf(line, sizeof(line), " [mv L<=%u %s%s frames=%u items=%u wg/cu=%u lds=%zu]", q.lmax,
+                          q.rf_dtype == SUPRA_INT16 ? "int16" : "float", q.means ? " means" : "", q.n_frames,
+                          q.n_items, q.wg_per_cu, q.lds);
+        b->config += line;
+    }
     if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(e, "batch: setup");
     if ((st = capture_graph(b)) != SUPRA_OK) return st;
     *out_batch = guard.release();

@@ -295,71 +295,40 @@ __device__ inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, ui
         __syncthreads();
     }
 
-    // 2. R(tau), 4x4 blocks of the upper triangle
-    const int nb = (L + 3) / 4;
-    const int nblk = nb * (nb + 1) / 2;
-    for (int item = threadIdx.x; item < H * nblk; item += kThreads) {
-        const int h = item / nblk;
-        int q = item % nblk, bi = 0;
-        while (q >= nb - bi) { q -= nb - bi; bi++; }
-        const int bj = bi + q;
-        const int i0 = 4 * bi, j0 = 4 * bj;
+    // 2. R(tau) per halo sample by diagonals: with x_k = x[f+k .. f+k+L_t),
+    //    R[i][j] = sum_{k < K_act} x[f+k+i] x[f+k+j] satisfies
+    //    R[i][j] = R[i-1][j-1] - x[f+i-1] x[f+j-1] + x[f+K_act-1+i] x[f+K_act-1+j],
+    //    so each diagonal d = j - i costs one K_act-term sum and two products
+    //    per further entry: O(L K + L^2) instead of O(L^2 K).  Products of two
+    //    floats are exact in double; the sums and the recurrence run in FP64,
+    //    so the stored single-precision entries carry one rounding each.
+    //    Means: sum (x_k - m)(x_k - m)^T = sum x_k x_k^T - K_act m m^T.
+    for (int item = threadIdx.x; item < H * L; item += kThreads) {
+        const int h = item / L, d = item % L;
         const SampleMeta m = meta[h];
-        float acc[4][4];
-#pragma unroll
-        for (int ii = 0; ii < 4; ii++)
-#pragma unroll
-            for (int jj = 0; jj < 4; jj++) acc[ii][jj] = 0.0f;
-        if (i0 < m.lt) {
-            const float* row = xs + h * rs + m.first;
-            float ma[4], mb[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                ma[u] = (MEANS && i0 + u < L) ? mean[h * L + i0 + u] : 0.0f;
-                mb[u] = (MEANS && j0 + u < L) ? mean[h * L + j0 + u] : 0.0f;
-            }
-            // k in steps of 4 with 7-sample windows (no register shifting);
-            // every entry still accumulates k = 0, 1, 2, ... in order
-            const int kact = m.kact;
-            int k = 0;
-            for (; k + 4 <= kact; k += 4) {
-                float va[7], vb[7];
-#pragma unroll
-                for (int u = 0; u < 7; u++) {
-                    va[u] = row[k + i0 + u];
-                    vb[u] = row[k + j0 + u];
+        const float* row = xs + h * rs + m.first;
+        float* dst = rp + h * a.packed;
+        const int lt = m.lt, kact = m.kact;
+        bool nonfinite = false;
+        int i = 0;
+        if (d < lt) {
+            double r = 0.0;
+            for (int k = 0; k < kact; k++) r = fma(static_cast<double>(row[k]), static_cast<double>(row[k + d]), r);
+            for (; i < lt - d; i++) {
+                if (i > 0) {
+                    r = fma(-static_cast<double>(row[i - 1]), static_cast<double>(row[i - 1 + d]), r);
+                    r = fma(static_cast<double>(row[kact - 1 + i]), static_cast<double>(row[kact - 1 + i + d]), r);
                 }
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-#pragma unroll
-                    for (int ii = 0; ii < 4; ii++) {
-                        const float av = va[q + ii] - ma[ii];
-#pragma unroll
-                        for (int jj = 0; jj < 4; jj++) acc[ii][jj] = fmaf(av, vb[q + jj] - mb[jj], acc[ii][jj]);
-                    }
-            }
-            for (; k < kact; k++) {
-#pragma unroll
-                for (int ii = 0; ii < 4; ii++) {
-                    const float av = row[k + i0 + ii] - ma[ii];
-#pragma unroll
-                    for (int jj = 0; jj < 4; jj++) acc[ii][jj] = fmaf(av, row[k + j0 + jj] - mb[jj], acc[ii][jj]);
-                }
+                double v = r;
+                if (MEANS)
+                    v -= static_cast<double>(kact) * static_cast<double>(mean[h * L + i]) *
+                         static_cast<double>(mean[h * L + i + d]);
+                const float fv = static_cast<float>(v);
+                nonfinite |= !(fabsf(fv) <= FLT_MAX);
+                dst[packed_index(i, i + d, L)] = fv;
             }
         }
-        float* dst = rp + h * a.packed;
-        bool nonfinite = false;
-#pragma unroll
-        for (int ii = 0; ii < 4; ii++)
-#pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
-                const int i = i0 + ii, j = j0 + jj;
-                if (i < L && j < L && j >= i) {
-                    const float v = (i < m.lt && j < m.lt) ? acc[ii][jj] : 0.0f;
-                    nonfinite |= !(fabsf(v) <= FLT_MAX);
-                    dst[packed_index(i, j, L)] = v;
-                }
-            }
+        for (; i + d < L; i++) dst[packed_index(i, i + d, L)] = 0.0f;
         // An overflowed or NaN covariance (e.g. from a -FLT_MAX pre-delayed
         // sample) makes every window holding the sample NaN in single
         // precision, hence output 0; it is kept out of the window sums.
@@ -519,7 +488,7 @@ hipError_t launch_io(const MvArgs& a, int in_dtype, int out_dtype, size_t lds, h
 
 // ---- persistent work queue over many frames (fused pre-delay + MV) --------
 
-template <int LMAX>
+template <typename InT, int LMAX, bool MEANS>
 __global__ __launch_bounds__(kThreads) void mv_batch_kernel(const MvBatchArgs b) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_item;
@@ -533,14 +502,7 @@ __global__ __launch_bounds__(kThreads) void mv_batch_kernel(const MvBatchArgs b)
         const MvBatchFrame& fr = b.frames[code >> 24];
         const MvArgs a = fr.a;     // scalars of the frame (the LUT is read from fr)
         const uint32_t local = code & 0xffffffu;
-        const uint32_t s = local / a.n_tiles, tile = local % a.n_tiles;
-        if (fr.rf_dtype == SUPRA_INT16) {
-            if (a.means) mv_tile<int16_t, float, LMAX, true, true>(a, fr.a.g.lut, s, tile, lds);
-            else mv_tile<int16_t, float, LMAX, false, true>(a, fr.a.g.lut, s, tile, lds);
-        } else {
-            if (a.means) mv_tile<float, float, LMAX, true, true>(a, fr.a.g.lut, s, tile, lds);
-            else mv_tile<float, float, LMAX, false, true>(a, fr.a.g.lut, s, tile, lds);
-        }
+        mv_tile<InT, float, LMAX, MEANS, true>(a, fr.a.g.lut, local / a.n_tiles, local % a.n_tiles, lds);
         __syncthreads();
         if (threadIdx.x == 0) {
             __threadfence();
@@ -650,9 +612,18 @@ int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, uint32_
     return SUPRA_OK;
 }
 
-hipError_t launch_mv_batch(const MvBatchArgs& b, uint32_t lmax, uint32_t workgroups, size_t lds, hipStream_t stream) {
-    if (lmax <= 32) hipLaunchKernelGGL(mv_batch_kernel<32>, dim3(workgroups), dim3(kThreads), lds, stream, b);
-    else hipLaunchKernelGGL(mv_batch_kernel<64>, dim3(workgroups), dim3(kThreads), lds, stream, b);
+hipError_t launch_mv_batch(const MvBatchArgs& b, uint32_t lmax, int32_t rf_dtype, int32_t means, uint32_t workgroups,
+                           size_t lds, hipStream_t stream) {
+    const dim3 grid(workgroups), block(kThreads);
+#define SUPRA_MV_BATCH(T, LM, M) hipLaunchKernelGGL((mv_batch_kernel<T, LM, M>), grid, block, lds, stream, b)
+    if (rf_dtype == SUPRA_INT16) {
+        if (lmax <= 32) { if (means) SUPRA_MV_BATCH(int16_t, 32, true); else SUPRA_MV_BATCH(int16_t, 32, false); }
+        else { if (means) SUPRA_MV_BATCH(int16_t, 64, true); else SUPRA_MV_BATCH(int16_t, 64, false); }
+    } else {
+        if (lmax <= 32) { if (means) SUPRA_MV_BATCH(float, 32, true); else SUPRA_MV_BATCH(float, 32, false); }
+        else { if (means) SUPRA_MV_BATCH(float, 64, true); else SUPRA_MV_BATCH(float, 64, false); }
+    }
+#undef SUPRA_MV_BATCH
     return hipGetLastError();
 }
 

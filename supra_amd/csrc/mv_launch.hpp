@@ -27,7 +27,7 @@ struct MvArgs {
 
 struct MvBatchFrame {
     MvArgs a;                   // fused mode: a.g holds the pre-delay of the frame
-    int32_t rf_dtype;
+    int32_t rf_dtype;           // (the queue's; kept for inspection)
     uint32_t n_items;           // work items (scanline x depth tile) of the frame
     uint32_t frame;             // index in the batch
     uint32_t pad;
@@ -46,8 +46,10 @@ int mv_prepare(MvArgs& a, uint32_t num_scanlines, uint32_t num_channels, uint32_
                uint32_t sub_array_size, uint32_t temporal_smoothing, double sub_array_scaling_power,
                int32_t compute_means, double loading_factor, size_t lds_budget, size_t* lds_out);
 
-// Persistent work-queue launch over a batch's minimum-variance frames (float output).
-hipError_t launch_mv_batch(const MvBatchArgs& b, uint32_t lmax, uint32_t workgroups, size_t lds, hipStream_t stream);
+// Persistent work-queue launch over a batch's minimum-variance frames of one
+// kernel class (sub-array bound, RF type, mean removal); float output.
+hipError_t launch_mv_batch(const MvBatchArgs& b, uint32_t lmax, int32_t rf_dtype, int32_t means, uint32_t workgroups,
+                           size_t lds, hipStream_t stream);
 
 // The pre-delay of `g` (2D DasArgs as for the pre-delay kernel: one frame,
 // n_ch output channels) feeding minimum variance over g.n_rx scanlines x
