@@ -278,13 +278,17 @@ int build_mv_queue(supra_batch* b, uint32_t qi, size_t budget) {
 }
 
 // LDS budgets of the MV queues: the whole LDS of a CU (one workgroup) or half
-// of it (two), less room for the kernel's static LDS.
+// of it (two), less room for the kernel's static LDS (16 bytes: the claimed
+// item's broadcast slot).
+constexpr size_t kMvStaticLdsReserve = 64;
 size_t mv_full_budget(const supra_batch* b) {
     int dev_lds = 65536;
     (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device);
-    return static_cast<size_t>(dev_lds) - 256;
+    return static_cast<size_t>(dev_lds) - kMvStaticLdsReserve;
 }
-size_t mv_half_budget(const supra_batch* b) { return (mv_full_budget(b) + 256) / 2 - 256; }
+size_t mv_half_budget(const supra_batch* b) {
+    return (mv_full_budget(b) + kMvStaticLdsReserve) / 2 - kMvStaticLdsReserve;
+}
 
 BatchSync make_sync(const supra_batch* b, uint32_t q) {
     BatchSync s;

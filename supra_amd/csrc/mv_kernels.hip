@@ -240,7 +240,7 @@ __device__ inline void stage_delayed(const MvArgs& a, const float* lut, uint32_t
 
 // One workgroup tile: output depths [tile * a.tile, ...) of scanline s.
 template <typename InT, typename OutT, int LMAX, bool MEANS, bool FUSED>
-__device__ inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, uint32_t tile, unsigned char* lds) {
+__device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, uint32_t tile, unsigned char* lds) {
     float* xs = reinterpret_cast<float*>(lds);
     SampleMeta* meta = reinterpret_cast<SampleMeta*>(lds + a.off_meta);
     float* mean = reinterpret_cast<float*>(lds + a.off_mean);
@@ -360,10 +360,26 @@ __device__ inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, ui
         // mirrored); the window loop is outside so that each step issues the
         // row's LDS loads back to back.  A window holding a non-finite
         // covariance (see phase 2) ends as NaN in single precision: output 0.
+        // Entries k >= n of a row, and the rows of lanes li >= n, are never
+        // read by the solve below, so the sums run unmasked over clamped
+        // in-range indices: one LDS load and one add per entry, no per-lane
+        // predicates to keep live across the window loop.
         float row[LMAX];
 #pragma unroll
         for (int k = 0; k < LMAX; k++) row[k] = 0.0f;
-        const int row_base = li * L - (li * (li - 1)) / 2 - li;   // packed_index(li, k) = row_base + k, k >= li
+        const int lic = min(li, L - 1);
+        int pidx[LMAX];
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) {
+            const int kc = min(k, L - 1);
+            const int i0 = min(lic, kc), j0 = max(lic, kc);
+            pidx[k] = i0 * L - (i0 * (i0 - 1)) / 2 + (j0 - i0);
+        }
+        // The diagonal entry is summed a second time on its own (same terms,
+        // same order, so diag == row[li] for li < n): extracting row[li]
+        // with a lane-dependent index would demote the row to scratch memory.
+        const int pdiag = lic * L - (lic * (lic - 1)) / 2;
+        float diag = 0.0f;
         int bad = 0;
         for (int h = w_lo; h <= w_hi; h++) {
             const SampleMeta mh = meta[h];
@@ -371,57 +387,48 @@ __device__ inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, ui
             if (mh.bad) continue;
             const float* src = rp + h * packed;
 #pragma unroll
-            for (int k = 0; k < LMAX; k++) {
-                if (k < n && li < n) {
-                    const int pi = (k >= li) ? row_base + k : (k * L - (k * (k - 1)) / 2 - k + li);
-                    row[k] += src[pi];
-                }
-            }
+            for (int k = 0; k < LMAX; k++) row[k] += src[pidx[k]];
+            diag += src[pdiag];
         }
-        float diag = 0.0f;
-#pragma unroll
-        for (int k = 0; k < LMAX; k++)
-            if (k == li) diag = row[k];
         const float trace = sub_sum<PPW>(li < n ? diag : 0.0f);
         const float delta = ((1.0f / static_cast<float>(n)) * trace) * a.alpha;
-#pragma unroll
-        for (int k = 0; k < LMAX; k++)
-            if (k == li) row[k] += delta;
+        // R~ row li with the loading on its diagonal (adding 0.0f elsewhere
+        // leaves the entries unchanged)
         float orig[LMAX];
 #pragma unroll
-        for (int k = 0; k < LMAX; k++) orig[k] = row[k];
-        // Cholesky, row `li` in registers, FP64 pivots and diagonal
-        double dd = 0.0;
-#pragma unroll
-        for (int k = 0; k < LMAX; k++)
-            if (k == li) dd = static_cast<double>(row[k]);
+        for (int k = 0; k < LMAX; k++) orig[k] = row[k] + ((k == li) ? delta : 0.0f);
+        // Cholesky, row `li` in registers, FP64 pivots and diagonal.  Row
+        // entries k >= li are never read by the factorisation, so the loading
+        // enters only through the diagonal dd.
+        double dd = static_cast<double>(diag + delta);
         float ginv = 1.0f;
         bool ok = true;
         // steps beyond the larger system of the wave are skipped (uniform branch)
         const int nmax = (PPW == 1) ? n : max(__builtin_amdgcn_readlane(n, 0), __builtin_amdgcn_readlane(n, 32));
 #pragma unroll
         for (int j = 0; j < LMAX; j++) {
-            if (j >= nmax) break;
-            const double piv = sub_bcast<PPW>(dd, j, half);
-            if (j < n) {
-                ok = ok && piv > 0.0;
-                // 1/sqrt(pivot) in FP64: hardware estimate and one Newton step
-                double r = __builtin_amdgcn_rsq(piv);
-                r = r * (1.5 - (0.5 * piv) * (r * r));
-                const float invg = static_cast<float>(r);
-                if (li == j) ginv = invg;
-                if (li > j && li < n) row[j] *= invg;
-                if (li > j && li < n) lmat[lrow + j] = row[j];
-            }
-            wave_sync();
-            // Branch-free update of the lanes below the pivot (others take a
-            // zero multiplier).  Entries k >= li of row li are its diagonal
-            // (tracked in dd) and upper part, which are never read again, so
-            // they are updated unmasked.
-            const float nlij = (j < n && li > j && li < n) ? -row[j] : 0.0f;
+            if (j < nmax) {   // a guard, not a break: the loop must unroll fully (row[j] in registers)
+                const double piv = sub_bcast<PPW>(dd, j, half);
+                if (j < n) {
+                    ok = ok && piv > 0.0;
+                    // 1/sqrt(pivot) in FP64: hardware estimate and one Newton step
+                    double r = __builtin_amdgcn_rsq(piv);
+                    r = r * (1.5 - (0.5 * piv) * (r * r));
+                    const float invg = static_cast<float>(r);
+                    if (li == j) ginv = invg;
+                    if (li > j && li < n) row[j] *= invg;
+                    if (li > j && li < n) lmat[lrow + j] = row[j];
+                }
+                wave_sync();
+                // Branch-free update of the lanes below the pivot (others take a
+                // zero multiplier).  Entries k >= li of row li are its diagonal
+                // (tracked in dd) and upper part, which are never read again, so
+                // they are updated unmasked.
+                const float nlij = (j < n && li > j && li < n) ? -row[j] : 0.0f;
 #pragma unroll
-            for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, lmat[(k * (k + 1)) / 2 + j], row[k]);
-            dd -= static_cast<double>(nlij) * static_cast<double>(nlij);
+                for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, lmat[(k * (k + 1)) / 2 + j], row[k]);
+                dd -= static_cast<double>(nlij) * static_cast<double>(nlij);
+            }
         }
         wave_sync();
         float w = chol_solve<LMAX, PPW>(row, ginv, 1.0f, n, li, half, lmat);
