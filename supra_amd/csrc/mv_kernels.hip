@@ -117,10 +117,12 @@ __device__ inline double sub_sum(double v) {
 // Solve (G G^T) v = b for the sub-wave's pixel: G's row i is in lane i's
 // registers a[] (strictly lower part), ginv is lane i's 1/G[i][i]; the
 // strictly lower factor is also in LDS (row-major packed, G[r][c] at
-// r (r + 1) / 2 + c) for the transposed solve.  Lanes i >= n carry no data.
+// r (r - 1) / 2 + c) for the transposed solve.  Lanes i >= n carry no data.
 template <int LMAX, int PPW>
 __device__ inline float chol_solve(const float (&a)[LMAX], float ginv, float b, int n, int li, int half,
                                    const float* lmat) {
+    // fresh lane-vs-step predicates for each call (not kept live across calls)
+    asm volatile("" : "+v"(li), "+v"(n));
     // forward: G z = b, column-oriented
     float c = b, z = 0.0f;
 #pragma unroll
@@ -136,7 +138,7 @@ __device__ inline float chol_solve(const float (&a)[LMAX], float ginv, float b, 
     for (int j = LMAX - 1; j >= 0; j--) {
         const float vj = sub_bcast<PPW>(c * ginv, j, half);
         if (li == j) v = vj;
-        if (li < j && j < n) c = fmaf(-lmat[(j * (j + 1)) / 2 + li], vj, c);
+        if (li < j && j < n) c = fmaf(-lmat[(j * (j - 1)) / 2 + li], vj, c);
     }
     return v;
 }
@@ -213,6 +215,7 @@ __device__ inline void stage_delayed(const MvArgs& a, const float* lut, uint32_t
     const RFT* rf_tx = static_cast<const RFT*>(g.rf) + static_cast<uint64_t>(tx.tx_idx) * g.n_ch * g.n_s;
     const __amdgpu_buffer_rsrc_t rsrc = rf_resource(rf_tx, g.n_ch * g.n_s * sizeof(RFT));
     const int rs = static_cast<int>(a.rs);
+#pragma unroll 4
     for (int idx = threadIdx.x; idx < H * rs; idx += kThreads) {
         const int c = idx / H, h = idx % H;
         float v = 0.0f;
@@ -239,6 +242,43 @@ __device__ inline void stage_delayed(const MvArgs& a, const float* lut, uint32_t
 }
 
 // One workgroup tile: output depths [tile * a.tile, ...) of scanline s.
+// Diagonals da <= db of one sample's R (db == da: one diagonal), entries
+// (i, i + d) for i < L - d, given each first entry's sum
+// r = sum_{k < K_act} x[f+k] x[f+k+d] (FP64): the rest by the recurrence,
+// minus K_act m_i m_{i+d} with means, single-precision stores; entries beyond
+// the sample's L_t are 0.  One loop walks diagonal da then diagonal db, so
+// the trip count (L_t + 1 entries for a pair) is the same on every lane that
+// holds a full-length sample.  Returns whether a stored entry is not finite.
+template <bool MEANS>
+__device__ inline bool cov_diagonals(const float* row, const float* mean, float* dst, int L, int lt, int kact, int da,
+                                     int db, double ra, double rb) {
+    bool nonfinite = false;
+    const bool two = db != da;
+    const int na = max(lt - da, 0), nb = two ? max(lt - db, 0) : 0;
+    double r = ra;
+    int d = da, i = 0;
+    for (int step = 0; step < na + nb; step++, i++) {
+        if (step == na) {   // on to the second diagonal
+            r = rb;
+            d = db;
+            i = 0;
+        }
+        if (i > 0) {
+            r = fma(-static_cast<double>(row[i - 1]), static_cast<double>(row[i - 1 + d]), r);
+            r = fma(static_cast<double>(row[kact - 1 + i]), static_cast<double>(row[kact - 1 + i + d]), r);
+        }
+        double v = r;
+        if (MEANS) v -= static_cast<double>(kact) * static_cast<double>(mean[i]) * static_cast<double>(mean[i + d]);
+        const float fv = static_cast<float>(v);
+        nonfinite |= !(fabsf(fv) <= FLT_MAX);
+        dst[packed_index(i, i + d, L)] = fv;
+    }
+    for (int j = na; j + da < L; j++) dst[packed_index(j, j + da, L)] = 0.0f;
+    if (two)
+        for (int j = nb; j + db < L; j++) dst[packed_index(j, j + db, L)] = 0.0f;
+    return nonfinite;
+}
+
 template <typename InT, typename OutT, int LMAX, bool MEANS, bool FUSED>
 __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, const float* lut, uint32_t s, uint32_t tile, unsigned char* lds) {
     float* xs = reinterpret_cast<float*>(lds);
@@ -259,26 +299,53 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
         stage_delayed<InT>(a, lut, s, h_lo, H, xs, lds);
     } else {
         const InT* xin = static_cast<const InT*>(a.x) + static_cast<uint64_t>(s) * a.n_ch * a.n_d;
-        for (int idx = threadIdx.x; idx < H * rs; idx += kThreads) {
-            const int c = idx / H, h = idx % H;
-            xs[h * rs + c] =
-                (c < static_cast<int>(a.n_ch)) ? in_f(xin[static_cast<uint64_t>(c) * a.n_d + h_lo + h]) : 0.0f;
+        // batches of 8 loads per lane in flight before the LDS stores (the
+        // loop is otherwise one global-memory round trip per element)
+        constexpr int kBatch = 8;
+        const int total = H * rs, n_ch = static_cast<int>(a.n_ch);
+        for (int base = 0; base < total; base += kBatch * kThreads) {
+            float v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const int idx = base + u * kThreads + static_cast<int>(threadIdx.x);
+                const int c = idx / H, h = idx % H;
+                v[u] = (idx < total && c < n_ch) ? in_f(xin[static_cast<uint64_t>(c) * a.n_d + h_lo + h]) : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const int idx = base + u * kThreads + static_cast<int>(threadIdx.x);
+                const int c = idx / H, h = idx % H;
+                if (idx < total) xs[h * rs + c] = v[u];
+            }
         }
         __syncthreads();
     }
-    for (int h = threadIdx.x; h < H; h += kThreads) {
-        const float* row = xs + h * rs;
-        int f = 0, l = 0;
-        for (int c = 0; c < static_cast<int>(a.n_ch); c++)
-            if (row[c] != 0.0f) { f = c; break; }
-        for (int c = static_cast<int>(a.n_ch) - 1; c >= 0; c--)
-            if (row[c] != 0.0f) { l = c; break; }
-        SampleMeta m;
-        m.first = f;
-        m.lt = min(L, l - f + 1);
-        m.kact = min(static_cast<int>(a.K), l - f - m.lt + 2);
-        m.bad = 0;
-        meta[h] = m;
+    // first / last channel with a non-zero sample (0 when there is none): one
+    // wave per sample, 64 channels per ballot
+    {
+        const int wv = threadIdx.x / 64, ln = threadIdx.x % 64;
+        const int n_ch = static_cast<int>(a.n_ch);
+        for (int h = wv; h < H; h += kWaves) {
+            const float* row = xs + h * rs;
+            int f = -1, l = 0;
+            for (int cb = 0; cb < n_ch; cb += 64) {
+                const int c = cb + ln;
+                const uint64_t nz = __ballot(c < n_ch && row[c] != 0.0f);
+                if (nz != 0) {
+                    if (f < 0) f = cb + __ffsll(static_cast<unsigned long long>(nz)) - 1;
+                    l = cb + 63 - __clzll(static_cast<long long>(nz));
+                }
+            }
+            if (f < 0) f = 0;
+            if (ln == 0) {
+                SampleMeta m;
+                m.first = f;
+                m.lt = min(L, l - f + 1);
+                m.kact = min(static_cast<int>(a.K), l - f - m.lt + 2);
+                m.bad = 0;
+                meta[h] = m;
+            }
+        }
     }
     __syncthreads();
     if (MEANS) {
@@ -303,32 +370,27 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
     //    floats are exact in double; the sums and the recurrence run in FP64,
     //    so the stored single-precision entries carry one rounding each.
     //    Means: sum (x_k - m)(x_k - m)^T = sum x_k x_k^T - K_act m m^T.
-    for (int item = threadIdx.x; item < H * L; item += kThreads) {
-        const int h = item / L, d = item % L;
+    //    Each lane takes the diagonal pair (d, L-1-d) of one sample, so every
+    //    lane walks about the same number of entries, and the two first-entry
+    //    sums share their loads of x[f+k] (two independent FMA chains).
+    const int pairs = (L + 1) / 2;
+    for (int item = threadIdx.x; item < H * pairs; item += kThreads) {
+        const int h = item / pairs, da = item % pairs, db = L - 1 - da;   // db == da: the middle diagonal
         const SampleMeta m = meta[h];
         const float* row = xs + h * rs + m.first;
         float* dst = rp + h * a.packed;
         const int lt = m.lt, kact = m.kact;
-        bool nonfinite = false;
-        int i = 0;
-        if (d < lt) {
-            double r = 0.0;
-            for (int k = 0; k < kact; k++) r = fma(static_cast<double>(row[k]), static_cast<double>(row[k + d]), r);
-            for (; i < lt - d; i++) {
-                if (i > 0) {
-                    r = fma(-static_cast<double>(row[i - 1]), static_cast<double>(row[i - 1 + d]), r);
-                    r = fma(static_cast<double>(row[kact - 1 + i]), static_cast<double>(row[kact - 1 + i + d]), r);
-                }
-                double v = r;
-                if (MEANS)
-                    v -= static_cast<double>(kact) * static_cast<double>(mean[h * L + i]) *
-                         static_cast<double>(mean[h * L + i + d]);
-                const float fv = static_cast<float>(v);
-                nonfinite |= !(fabsf(fv) <= FLT_MAX);
-                dst[packed_index(i, i + d, L)] = fv;
+        double ra = 0.0, rb = 0.0;
+        if (db < lt) {
+            for (int k = 0; k < kact; k++) {
+                const double xk = static_cast<double>(row[k]);
+                ra = fma(xk, static_cast<double>(row[k + da]), ra);
+                rb = fma(xk, static_cast<double>(row[k + db]), rb);
             }
+        } else if (da < lt) {
+            for (int k = 0; k < kact; k++) ra = fma(static_cast<double>(row[k]), static_cast<double>(row[k + da]), ra);
         }
-        for (; i + d < L; i++) dst[packed_index(i, i + d, L)] = 0.0f;
+        const bool nonfinite = cov_diagonals<MEANS>(row, mean + h * L, dst, L, lt, kact, da, db, ra, rb);
         // An overflowed or NaN covariance (e.g. from a -FLT_MAX pre-delayed
         // sample) makes every window holding the sample NaN in single
         // precision, hence output 0; it is kept out of the window sums.
@@ -343,12 +405,17 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (wave >= static_cast<int>(a.waves)) return;   // (no barrier follows in this tile)
     const int half = (PPW == 2) ? lane / 32 : 0;
-    const int li = (PPW == 2) ? lane % 32 : lane;
-    // strictly lower factor, row-major packed: G[r][c] at r (r + 1) / 2 + c
+    const int li_lane = (PPW == 2) ? lane % 32 : lane;
+    // strictly lower factor, row-major packed: G[r][c] (c < r) at r (r - 1) / 2 + c
     float* lmat = reinterpret_cast<float*>(lds + a.off_wave + (wave * PPW + half) * a.wave_bytes);
-    const int lrow = (li * (li + 1)) / 2;
+    const int lrow_lane = (li_lane * (li_lane - 1)) / 2;
     OutT* out = static_cast<OutT*>(a.out);
     for (uint32_t tb = t0 + wave * PPW; tb < t1; tb += a.waves * PPW) {
+        // The lane index is made opaque per sample: otherwise the compiler
+        // hoists every lane-vs-step comparison of the unrolled factorisation
+        // and solves out of this loop as a 64-bit mask each, and spills them.
+        int li = li_lane, lrow = lrow_lane;
+        asm volatile("" : "+v"(li), "+v"(lrow));
         const bool valid = tb + half < t1;
         const uint32_t t = valid ? tb + half : t1 - 1;    // a spare half repeats the last sample
         const int ht = static_cast<int>(t - h_lo);
@@ -419,14 +486,25 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
                     if (li > j && li < n) row[j] *= invg;
                     if (li > j && li < n) lmat[lrow + j] = row[j];
                 }
-                wave_sync();
                 // Branch-free update of the lanes below the pivot (others take a
                 // zero multiplier).  Entries k >= li of row li are its diagonal
                 // (tracked in dd) and upper part, which are never read again, so
-                // they are updated unmasked.
+                // they are updated unmasked.  G[k][j] is lane k's row[j]: with
+                // one system per wave it is broadcast straight from that lane
+                // (no LDS round trip per step); two systems per wave read it
+                // back from the factor in LDS.
                 const float nlij = (j < n && li > j && li < n) ? -row[j] : 0.0f;
+                if constexpr (PPW == 1) {
+                    float gcol[LMAX];
 #pragma unroll
-                for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, lmat[(k * (k + 1)) / 2 + j], row[k]);
+                    for (int k = j + 1; k < LMAX; k++) gcol[k] = bcast(row[j], k);
+#pragma unroll
+                    for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, gcol[k], row[k]);
+                } else {
+                    wave_sync();
+#pragma unroll
+                    for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, lmat[(k * (k - 1)) / 2 + j], row[k]);
+                }
                 dd -= static_cast<double>(nlij) * static_cast<double>(nlij);
             }
         }
@@ -545,7 +623,7 @@ size_t plan_lds(MvArgs& a, uint32_t tile) {
     off += static_cast<size_t>(H) * a.packed * 4;
     off = (off + 15) & ~static_cast<size_t>(15);
     a.off_wave = static_cast<uint32_t>(off);
-    a.wave_bytes = static_cast<uint32_t>(((a.L * (a.L + 1) / 2 * 4) + 15) & ~15u);
+    a.wave_bytes = static_cast<uint32_t>(((a.L * (a.L - 1) / 2 * 4) + 15) & ~15u);   // strictly lower factor
     const uint32_t ppw = a.L <= 32 ? 2 : 1;   // pixels per wave in phase 3
     off += static_cast<size_t>(a.waves) * ppw * a.wave_bytes;
     a.off_fused = static_cast<uint32_t>(off);
