@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cfloat>
 #include <cmath>
 #include <cstddef>
@@ -129,16 +131,28 @@ template <> __device__ inline float load_rf<float>(__amdgpu_buffer_rsrc_t r, uin
 template <typename RFT, bool SAT_UINT>
 __device__ inline void gather(__amdgpu_buffer_rsrc_t rf, uint32_t off, uint32_t n_s, float tau, float w,
                               float& sample) {
-    const float fl = floorf(tau);
     uint32_t ui;
     float f;
-    if (SAT_UINT) {
+    if constexpr (SAT_UINT) {
+        const float fl = floorf(tau);
         ui = (fl > 0.0f) ? static_cast<uint32_t>(fl) : 0u;
         f = tau - static_cast<float>(ui);
     } else {
-        const int32_t i = static_cast<int32_t>(fl);
-        ui = static_cast<uint32_t>(i);
-        f = tau - static_cast<float>(i);
+        // tau - floor(tau) by v_fract_f32: exact for tau >= 0, the only case
+        // that can be in range
+        ui = static_cast<uint32_t>(static_cast<int32_t>(floorf(tau)));
+        f = __builtin_amdgcn_fractf(tau);
+    }
+    if constexpr (!SAT_UINT && std::is_same_v<RFT, int16_t>) {
+        // 2D over int16 RF: floor(tau) < N-1, or == N-1 with zero fraction,
+        // is 0 <= tau <= N-1; at tau == N-1 the neighbour sample is finite,
+        // so (1 - 0) v0 + 0 v1 is v0 exactly and needs no separate case.
+        const float v0 = load_rf<RFT>(rf, off + ui);
+        const float v1 = load_rf<RFT>(rf, off + ui + 1u);
+        const bool in = (tau >= 0.0f) & (tau <= static_cast<float>(n_s - 1u));
+        const float contrib = w * ((1.0f - f) * v0 + f * v1);
+        sample += in ? contrib : 0.0f;   // (a select: keeps the loads unconditional, no branch per element)
+        return;
     }
     const float v0 = load_rf<RFT>(rf, off + ui);
     const float v1 = load_rf<RFT>(rf, off + ui + 1u);
