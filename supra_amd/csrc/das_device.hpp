@@ -134,22 +134,32 @@ __device__ inline void gather(__amdgpu_buffer_rsrc_t rf, uint32_t off, uint32_t 
     uint32_t ui;
     float f;
     if constexpr (SAT_UINT) {
-        const float fl = floorf(tau);
-        ui = (fl > 0.0f) ? static_cast<uint32_t>(fl) : 0u;
-        f = tau - static_cast<float>(ui);
+        // floor(tau) clamped at 0 (NaN -> 0 like the compare-and-select),
+        // and tau minus that same float (equal to tau - float(ui) wherever
+        // ui can be in range)
+        const float fl = fmaxf(floorf(tau), 0.0f);
+        ui = static_cast<uint32_t>(fl);
+        f = tau - fl;
     } else {
         // tau - floor(tau) by v_fract_f32: exact for tau >= 0, the only case
         // that can be in range
         ui = static_cast<uint32_t>(static_cast<int32_t>(floorf(tau)));
         f = __builtin_amdgcn_fractf(tau);
     }
-    if constexpr (!SAT_UINT && std::is_same_v<RFT, int16_t>) {
-        // 2D over int16 RF: floor(tau) < N-1, or == N-1 with zero fraction,
-        // is 0 <= tau <= N-1; at tau == N-1 the neighbour sample is finite,
-        // so (1 - 0) v0 + 0 v1 is v0 exactly and needs no separate case.
+    if constexpr (std::is_same_v<RFT, int16_t>) {
+        // int16 RF: "ui < N-1, or ui == N-1 with zero fraction" is a range
+        // test on tau itself, and at tau == N-1 the neighbour sample is
+        // finite, so (1 - 0) v0 + 0 v1 is v0 exactly and needs no separate
+        // case.  2D (int32 floor): 0 <= tau <= N-1.  3D (floor saturated at
+        // 0, so a negative tau interpolates from sample 0): tau <= N-1, or
+        // tau == 0 when N == 1; a NaN delay, which the saturated floor maps
+        // to sample 0, stays in range (for N > 1) as before.
         const float v0 = load_rf<RFT>(rf, off + ui);
         const float v1 = load_rf<RFT>(rf, off + ui + 1u);
-        const bool in = (tau >= 0.0f) & (tau <= static_cast<float>(n_s - 1u));
+        const float last = static_cast<float>(n_s - 1u);
+        bool in;
+        if constexpr (SAT_UINT) in = !(tau > last) & !(tau < (n_s > 1u ? -INFINITY : 0.0f));
+        else in = (tau >= 0.0f) & (tau <= last);
         const float contrib = w * ((1.0f - f) * v0 + f * v1);
         sample += in ? contrib : 0.0f;   // (a select: keeps the loads unconditional, no branch per element)
         return;
