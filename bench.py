@@ -114,13 +114,39 @@ def bench_2d(args, torch, frames: int = 32):
     samples = g.num_rx * g.num_depths
     gbs = das_bytes(w, frames) / (ev_ms * 1e-3) / 1e9
     bf.close()
-    return {
+    res = {
         "workload": f"{w.name}: 128x2080 image, int16 RF 128x64x2080, {frames} frames per launch",
         "samples_per_s": samples / (per_frame_ms * 1e-3),
         "us_per_frame": per_frame_ms * 1e3,
         "achieved_GBs": gbs,
         "hbm_frac": gbs / HBM_PEAK_GBS,
     }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_2d(w, g, rf, out, kw, args.cpu_budget / 3)
+        res["gpu_over_cpu"] = res["samples_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
+def cpu_baseline_2d(w, g, rf, out, kw, budget_s: float):
+    """Host FP32 oracle on whole frames of the same 2D batch (one frame after
+    another until budget_s has passed), each checked against the GPU image."""
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    rf_host = rf.cpu().numpy()
+    out_host = out.cpu().numpy()
+    done, exact, elapsed = 0, True, 0.0
+    while done < rf_host.shape[0] and (done == 0 or elapsed < budget_s):
+        t0 = time.perf_counter()
+        img = O.das(g, rf_host[done], w.fs_hz, threads=threads, **kw)
+        elapsed += time.perf_counter() - t0
+        exact = exact and bool(np.array_equal(img, out_host[done]))
+        done += 1
+    return {"value": done * g.num_rx * g.num_depths / elapsed, "unit": "samples/s", "cores": threads,
+            "kind": "port", "sample": f"{done} whole frames ({done * g.num_rx * g.num_depths} samples, "
+                                      f"{elapsed:.1f} s)",
+            "gpu_matches_sample_bit_exact": exact}
 
 
 def bench_diff(args, torch, frames: int = 16):
