@@ -46,8 +46,17 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kChunk2D = 512;     // aperture elements staged per LDS pass (2D)
-constexpr int kChunk3D = 512;     // aperture elements staged per LDS pass (3D)
-constexpr int kW2Stride = kWindowEntries + 1;   // padded rows: spread LDS banks
+// Aperture elements staged per LDS pass (3D).  Small on purpose: the 3D
+// kernel is bound by gather latency, and a small LDS footprint per
+// workgroup (about 12 KB with the packed window table) lets 10 workgroups
+// share a CU.
+constexpr int kChunk3D = 128;
+// sqrt(LUT[i] * LUT[j]) is symmetric in (i, j): upper triangle, row-major
+constexpr int kW2Entries = kWindowEntries * (kWindowEntries + 1) / 2;
+__device__ inline int w2_index(int i, int j) {
+    const int lo = min(i, j), hi = max(i, j);
+    return lo * kWindowEntries - (lo * (lo - 1)) / 2 + (hi - lo);
+}
 constexpr uint32_t kNotConnected = 0xffffffffu;
 
 struct Pixel2D {
@@ -260,7 +269,7 @@ __device__ inline void elements3d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float
                                   const float* s_xe, const float* s_ze, const float* s_dx, const float* s_dz,
                                   const float* s_r2, const float* s_w, const uint32_t* s_off, const float* s_w2,
                                   int cnt, float& sample, float& wsum, int& nadd) {
-#pragma unroll 4
+#pragma unroll 8
     for (int j = 0; j < cnt; j++) {
         const uint32_t off = s_off[j];
         if (MAP && off == kNotConnected) continue;
@@ -271,7 +280,7 @@ __device__ inline void elements3d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float
             if (!(s_r2[j] <= p.aDT)) continue;
             const int ix = window_index(s_dx[j] * p.invx);
             const int iz = window_index(s_dz[j] * p.invz);
-            w = s_w2[ix * kW2Stride + iz];
+            w = s_w2[w2_index(ix, iz)];
         }
         wsum += w;
         nadd++;
@@ -301,7 +310,7 @@ __device__ inline float test_signal_3d(const ScanView& sc, float d, float dt, fl
 
 template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
 __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
-    __shared__ float s_w2[kWindowEntries * kW2Stride];   // sqrt(LUT[i] * LUT[j]), padded rows
+    __shared__ float s_w2[kW2Entries];   // sqrt(LUT[i] * LUT[j]), packed upper triangle
     __shared__ float s_xe[kChunk3D];
     __shared__ float s_ze[kChunk3D];
     __shared__ float s_dx[kChunk3D];
@@ -315,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
     // the product and the square root are the same IEEE operations.
     for (int i = threadIdx.x; i < kWindowEntries * kWindowEntries; i += kThreads) {
         const int ix = i / kWindowEntries, iz = i % kWindowEntries;
-        s_w2[ix * kW2Stride + iz] = sqrtf(a.lut[ix] * a.lut[iz]);
+        if (iz >= ix) s_w2[w2_index(ix, iz)] = sqrtf(a.lut[ix] * a.lut[iz]);
     }
 
     const ScanView sc = scan_view(a.scans, t.s);
@@ -382,7 +391,7 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
                     s_dx[j] = dx;
                     s_dz[j] = dz;
                     s_r2[j] = dx * dx + dz * dz;
-                    s_w[j] = s_w2[window_index(dx * invx_far) * kW2Stride + window_index(dz * invz_far)];
+                    s_w[j] = s_w2[w2_index(window_index(dx * invx_far), window_index(dz * invz_far))];
                     s_off[j] = (ch == kNotConnected) ? kNotConnected : ch * a.n_s;
                 }
                 __syncthreads();
