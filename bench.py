@@ -229,8 +229,10 @@ def bench_mv(args, torch):
     torch.linalg.solve)."""
     from supra_amd import _native as N
     from supra_amd import workloads as W
-    from supra_amd.beamformer import RawDelay, USRawData, mv_perform_rx_beamforming
+    from supra_amd.beamformer import (RawDelay, RxBeamformer, USRawData, mv_das_perform_rx_beamforming,
+                                      mv_perform_rx_beamforming)
     sys.path.insert(0, str(ROOT / "tests"))
+    from torch_das_reference import PlainTorchDAS
     from torch_mv_reference import plain_mv
 
     w = W.linear_2d()
@@ -249,14 +251,26 @@ def bench_mv(args, torch):
     def delay_step():
         rd.perform_delay(raw, kw["f_number"], kw["speed_of_sound_mm_per_s"], N.WINDOW_HANN, 0.0, out=delayed.data)
 
-    log("[bench] timing minimum variance (fused kernel) and the pre-delay")
+    engine = RxBeamformer(g)
+
+    def fused_step():
+        mv_das_perform_rx_beamforming(engine, raw, kw["f_number"], kw["speed_of_sound_mm_per_s"], N.WINDOW_HANN,
+                                      0.0, MV_L, MV_T, MV_P, False, MV_ALPHA, out=out)
+
+    log("[bench] timing minimum variance, the pre-delay, and the fused RF -> image launch")
     _, mv_ms = time_launches(mv_step, max(5, args.steps), 2, torch)
     _, rd_ms = time_launches(delay_step, max(5, args.steps), 2, torch)
-    log("[bench] timing the plain PyTorch minimum variance")
+    _, fused_ms = time_launches(fused_step, max(5, args.steps), 2, torch)
+    log("[bench] timing the plain PyTorch pre-delay and minimum variance")
     _, plain_ms = time_launches(lambda: plain_mv(delayed.data, MV_L, MV_T, MV_P, MV_ALPHA), 3, 1, torch)
+    pdas = PlainTorchDAS(g, w.fs_hz, kw["f_number"], N.WINDOW_HANN, 0.0, w.num_channels, w.rf_shape[2])
+    rf32 = rf.to(torch.float32)
+    c_t = torch.tensor(kw["speed_of_sound_mm_per_s"] / 1000.0, device="cuda")
+    _, plain_rd_ms = time_launches(lambda: pdas.delayed(rf32, c_t, w.num_channels), 3, 1, torch)
     pixels = g.num_rx * g.num_depths
     flops = mv_flops_per_pixel(w.num_channels, MV_L, MV_T) * pixels
     rd.close()
+    engine.close()
     return {
         "workload": "linear_2d frame -> RawDelay (F#1 Hann) -> MV L=32 T=5 loading trace/(100L), "
                     "128x2080 samples",
@@ -265,6 +279,13 @@ def bench_mv(args, torch):
         "rawdelay_ms": rd_ms,
         "plain_torch_mv_ms": plain_ms,
         "speedup_vs_plain": plain_ms / mv_ms,
+        "rf_to_image": {
+            "fused_ms": fused_ms,
+            "two_step_ms": rd_ms + mv_ms,
+            "plain_torch_ms": plain_rd_ms + plain_ms,
+            "plain_torch_rawdelay_ms": plain_rd_ms,
+            "speedup_vs_plain": (plain_rd_ms + plain_ms) / fused_ms,
+        },
         "algorithmic_flop": flops,
         "achieved_TFLOPs": flops / (mv_ms * 1e-3) / 1e12,
         "fp32_frac": flops / (mv_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,

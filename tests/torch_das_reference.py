@@ -81,3 +81,22 @@ class PlainTorchDAS:
         v1 = flat[idx + 1]
         val = ((1.0 - f) * v0 + f * v1) * valid
         return (self.wn * val).sum(-1).transpose(0, 1)
+
+    def delayed(self, rf_frame, c_m_per_s, num_channels):
+        """The pre-delay of RawDelay (RawDelay.cu:221-312) in the same plain
+        form: per aperture element j = e - first, w_e (n/W) interp(tau)
+        -> [scanline][num_channels][depth] (channels beyond the aperture 0)."""
+        scale = self.fs / (c_m_per_s.to(torch.float32) * 1000.0)
+        tau = self.t0[:, None, None] + self.path * scale
+        i = torch.floor(tau)
+        f = tau - i
+        ii = i.to(torch.int64)
+        valid = (ii >= 0) & (ii < self.n_s - 1)
+        idx = self.line[:, None, :] + ii.clamp(0, self.n_s - 2)
+        flat = rf_frame.reshape(-1)
+        val = ((1.0 - f) * flat[idx] + f * flat[idx + 1]) * valid
+        d = (self.wn * val).permute(0, 2, 1)                           # [rx][A][D]
+        out = torch.zeros((self.n_rx, num_channels, self.n_d), dtype=torch.float32, device=d.device)
+        a = min(d.shape[1], num_channels)
+        out[:, :a] = d[:, :a]
+        return out
