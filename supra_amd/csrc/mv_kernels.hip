@@ -103,6 +103,57 @@ __device__ inline double sub_bcast(double v, int j, int half) {
     const double a0 = bcast(v, j), a1 = bcast(v, j + 32);
     return half ? a1 : a0;
 }
+// Broadcast of lane j (0..31) within each half-wave by ds_swizzle (bit-mask
+// mode: and 0, or j), one instruction instead of two readlanes and a select;
+// j must fold to a constant (unrolled loops).
+template <int J> __device__ inline int swizzle_from(int v) { return __builtin_amdgcn_ds_swizzle(v, J << 5); }
+__device__ inline float half_bcast(float v, int j) {
+    const int x = __float_as_int(v);
+    int r = 0;
+    switch (j) {
+        case 0: r = swizzle_from<0>(x); break;
+        case 1: r = swizzle_from<1>(x); break;
+        case 2: r = swizzle_from<2>(x); break;
+        case 3: r = swizzle_from<3>(x); break;
+        case 4: r = swizzle_from<4>(x); break;
+        case 5: r = swizzle_from<5>(x); break;
+        case 6: r = swizzle_from<6>(x); break;
+        case 7: r = swizzle_from<7>(x); break;
+        case 8: r = swizzle_from<8>(x); break;
+        case 9: r = swizzle_from<9>(x); break;
+        case 10: r = swizzle_from<10>(x); break;
+        case 11: r = swizzle_from<11>(x); break;
+        case 12: r = swizzle_from<12>(x); break;
+        case 13: r = swizzle_from<13>(x); break;
+        case 14: r = swizzle_from<14>(x); break;
+        case 15: r = swizzle_from<15>(x); break;
+        case 16: r = swizzle_from<16>(x); break;
+        case 17: r = swizzle_from<17>(x); break;
+        case 18: r = swizzle_from<18>(x); break;
+        case 19: r = swizzle_from<19>(x); break;
+        case 20: r = swizzle_from<20>(x); break;
+        case 21: r = swizzle_from<21>(x); break;
+        case 22: r = swizzle_from<22>(x); break;
+        case 23: r = swizzle_from<23>(x); break;
+        case 24: r = swizzle_from<24>(x); break;
+        case 25: r = swizzle_from<25>(x); break;
+        case 26: r = swizzle_from<26>(x); break;
+        case 27: r = swizzle_from<27>(x); break;
+        case 28: r = swizzle_from<28>(x); break;
+        case 29: r = swizzle_from<29>(x); break;
+        case 30: r = swizzle_from<30>(x); break;
+        case 31: r = swizzle_from<31>(x); break;
+    }
+    return __int_as_float(r);
+}
+// Broadcast for the throughput-bound loops (solves, residual): ds_swizzle
+// with two systems per wave, readlane with one.
+template <int PPW>
+__device__ inline float sub_bcast_swz(float v, int j, int half) {
+    if (PPW == 1) return bcast(v, j);
+    (void)half;
+    return half_bcast(v, j);
+}
 template <int PPW>
 __device__ inline float sub_sum(float v) {
     for (int m = 32 / PPW; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -127,7 +178,7 @@ __device__ inline float chol_solve(const float (&a)[LMAX], float ginv, float b, 
     float c = b, z = 0.0f;
 #pragma unroll
     for (int j = 0; j < LMAX; j++) {
-        const float zj = sub_bcast<PPW>(c * ginv, j, half);
+        const float zj = sub_bcast_swz<PPW>(c * ginv, j, half);
         if (li == j) z = zj;
         if (li > j && li < n && j < n) c = fmaf(-a[j], zj, c);
     }
@@ -136,7 +187,7 @@ __device__ inline float chol_solve(const float (&a)[LMAX], float ginv, float b, 
     c = z;
 #pragma unroll
     for (int j = LMAX - 1; j >= 0; j--) {
-        const float vj = sub_bcast<PPW>(c * ginv, j, half);
+        const float vj = sub_bcast_swz<PPW>(c * ginv, j, half);
         if (li == j) v = vj;
         if (li < j && j < n) c = fmaf(-lmat[(j * (j - 1)) / 2 + li], vj, c);
     }
@@ -468,21 +519,18 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
         // entries k >= li are never read by the factorisation, so the loading
         // enters only through the diagonal dd.
         double dd = static_cast<double>(diag + delta);
-        float ginv = 1.0f;
-        bool ok = true;
         // steps beyond the larger system of the wave are skipped (uniform branch)
         const int nmax = (PPW == 1) ? n : max(__builtin_amdgcn_readlane(n, 0), __builtin_amdgcn_readlane(n, 32));
 #pragma unroll
         for (int j = 0; j < LMAX; j++) {
             if (j < nmax) {   // a guard, not a break: the loop must unroll fully (row[j] in registers)
-                const double piv = sub_bcast<PPW>(dd, j, half);
+                // 1/sqrt of the lane's own running diagonal in FP64 (hardware
+                // estimate and one Newton step); at step j lane j's value is
+                // the pivot's, broadcast as the float the rows are scaled by
+                double r = __builtin_amdgcn_rsq(dd);
+                r = r * (1.5 - (0.5 * dd) * (r * r));
+                const float invg = sub_bcast<PPW>(static_cast<float>(r), j, half);
                 if (j < n) {
-                    ok = ok && piv > 0.0;
-                    // 1/sqrt(pivot) in FP64: hardware estimate and one Newton step
-                    double r = __builtin_amdgcn_rsq(piv);
-                    r = r * (1.5 - (0.5 * piv) * (r * r));
-                    const float invg = static_cast<float>(r);
-                    if (li == j) ginv = invg;
                     if (li > j && li < n) row[j] *= invg;
                     if (li > j && li < n) lmat[lrow + j] = row[j];
                 }
@@ -508,6 +556,14 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
                 dd -= static_cast<double>(nlij) * static_cast<double>(nlij);
             }
         }
+        // Row li's diagonal stops changing after step li - 1, so dd is now
+        // lane li's pivot: its 1/G[li][li] (the value broadcast at step li)
+        // and whether every pivot of the system was positive.
+        double rg = __builtin_amdgcn_rsq(dd);
+        rg = rg * (1.5 - (0.5 * dd) * (rg * rg));
+        const float ginv = static_cast<float>(rg);
+        const uint64_t nonpos = __ballot(li < n && !(dd > 0.0));
+        const bool ok = (PPW == 1) ? nonpos == 0 : ((nonpos >> (32 * half)) & 0xffffffffull) == 0;
         wave_sync();
         float w = chol_solve<LMAX, PPW>(row, ginv, 1.0f, n, li, half, lmat);
         // FP64 residual r = R~ w - 1, up to two refinement steps
@@ -515,7 +571,7 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
             double r = 0.0;
 #pragma unroll
             for (int k = 0; k < LMAX; k++) {
-                const double wk = static_cast<double>(sub_bcast<PPW>(w, k, half));
+                const double wk = static_cast<double>(sub_bcast_swz<PPW>(w, k, half));
                 if (k < n) r += static_cast<double>(orig[k]) * wk;
             }
             r = (li < n) ? r - 1.0 : 0.0;
