@@ -45,7 +45,12 @@ namespace supra_amd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kChunk2D = 512;     // aperture elements staged per LDS pass (2D)
+constexpr int kChunk2D = 128;     // aperture elements staged per LDS pass (2D, per wave)
+// 2D work unit: one wave = 64 consecutive depths of one scanline (a 2080-deep
+// image is 32.5 such units; with 256-depth workgroup tiles it would be 8.1
+// tiles, a tenth of the lanes idle in the last one).
+constexpr int kWaveDepths = 64;
+constexpr int kWavesPerGroup = kThreads / 64;
 // Aperture elements staged per LDS pass (3D).  Small on purpose: the 3D
 // kernel is bound by gather latency, and a small LDS footprint per
 // workgroup (about 12 KB with the packed window table) lets 10 workgroups
@@ -128,7 +133,29 @@ __device__ inline Tile decode_tile(const DasArgs& a) {
     return t;
 }
 
-// LDS working set of one 2D tile.
+// The 2D kernel's unit: wave w of workgroup b takes unit b * 4 + w of the
+// (frame, scanline, 64-depth chunk) list; units past the end are invalid.
+__device__ inline Tile decode_wave_2d(const DasArgs& a) {
+    const uint32_t chunks = (a.n_depths + kWaveDepths - 1) / kWaveDepths;
+    const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
+    const uint64_t unit = static_cast<uint64_t>(blockIdx.x) * kWavesPerGroup + wave;
+    const uint64_t rest = unit / chunks;
+    Tile t;
+    t.s = static_cast<uint32_t>(rest % a.n_rx);
+    t.frame = static_cast<uint32_t>(rest / a.n_rx);
+    t.r0 = static_cast<uint32_t>(unit % chunks) * kWaveDepths;
+    t.r = t.r0 + threadIdx.x % 64;
+    t.valid = rest < static_cast<uint64_t>(a.n_rx) * a.n_frames && t.r < a.n_depths;
+    return t;
+}
+
+__device__ inline void wave_sync_2d() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS working set of one wave's 2D unit.
 struct Lds2D {
     float lut[kWindowEntries];
     float xe[kChunk2D];
@@ -138,16 +165,19 @@ struct Lds2D {
     uint32_t off[kChunk2D];
 };
 
-// One 2D tile: 256 consecutive depths of one scanline of one frame.
+// One wave's 2D unit: 64 consecutive depths (t.r0 ...) of one scanline of one
+// frame, with the wave's own LDS tables (no workgroup barrier).
 template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
 __device__ inline void das2d_tile(const DasArgs& a, const float* lut, const Tile& t, Lds2D& L) {
+    if (!__any(t.valid)) return;   // (wave-uniform) a unit past the end of the image or the list
+    const int lane = static_cast<int>(threadIdx.x % 64);
     float* s_lut = L.lut;
     float* s_xe = L.xe;
     float* s_dx = L.dx;
     float* s_dx2 = L.dx2;
     float* s_w = L.w;
     uint32_t* s_off = L.off;
-    for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = lut[i];
+    for (int i = lane; i < kWindowEntries; i += 64) s_lut[i] = lut[i];
 
     // Per-frame tables (strides are 0 when every frame shares one geometry).
     const ScanlineDT* scans = a.scans + static_cast<size_t>(t.frame) * a.scan_frame_stride;
@@ -196,8 +226,8 @@ __device__ inline void das2d_tile(const DasArgs& a, const float* lut, const Tile
             const int first = tx.first_x, last = tx.last_x;
             for (int base = first; base < last; base += kChunk2D) {
                 const int cnt = min(kChunk2D, last - base);
-                __syncthreads();
-                for (int j = threadIdx.x; j < cnt; j += kThreads) {
+                wave_sync_2d();
+                for (int j = lane; j < cnt; j += 64) {
                     const int e = base + j;
                     uint32_t ch;
                     if (MAP) {
@@ -214,7 +244,7 @@ __device__ inline void das2d_tile(const DasArgs& a, const float* lut, const Tile
                     s_w[j] = s_lut[window_index(dx * inv_far)];
                     s_off[j] = (ch == kNotConnected) ? kNotConnected : ch * a.n_s;
                 }
-                __syncthreads();
+                wave_sync_2d();
                 if (far) {
                     if (vertical)
                         elements2d<RFT, true, true, MAP>(rsrc, a.n_s, tx.t0, a.offset, p, s_xe, s_dx, s_dx2, s_w,
@@ -252,8 +282,9 @@ __device__ inline void das2d_tile(const DasArgs& a, const float* lut, const Tile
 
 template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
 __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
-    __shared__ Lds2D lds;
-    das2d_tile<RFT, OutT, TESTSIG, INTERP_TX, MAP>(a, a.lut, decode_tile(a), lds);
+    __shared__ Lds2D lds[kWavesPerGroup];
+    const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
+    das2d_tile<RFT, OutT, TESTSIG, INTERP_TX, MAP>(a, a.lut, decode_wave_2d(a), lds[wave]);
 }
 
 // ---- 3D ---------------------------------------------------------------------
@@ -522,8 +553,10 @@ __device__ inline void das2d_batch_tile(const DasBatchFrame& fr, const DasArgs& 
 // queue is empty; the last item of a frame records the frame's completion
 // time.  Every workgroup leaves the loop once the queue index passes n_items.
 __global__ __launch_bounds__(kThreads) void das2d_batch_kernel(const DasBatchArgs b) {
-    __shared__ Lds2D lds;
+    __shared__ Lds2D lds_waves[kWavesPerGroup];
     __shared__ uint32_t s_item;
+    const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
+    Lds2D& lds = lds_waves[wave];
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(b.sync.head, 1u);
@@ -538,8 +571,8 @@ __global__ __launch_bounds__(kThreads) void das2d_batch_kernel(const DasBatchArg
         const uint32_t tile = local % a.depth_tiles;
         t.s = local / a.depth_tiles;
         t.frame = 0;
-        t.r0 = tile * kThreads;
-        t.r = t.r0 + threadIdx.x;
+        t.r0 = tile * kThreads + wave * kWaveDepths;   // each wave: 64 depths of the item's 256
+        t.r = t.r0 + threadIdx.x % 64;
         t.valid = t.r < a.n_depths;
         if (fr.rf_dtype == SUPRA_INT16) {
             if (fr.out_dtype == SUPRA_FLOAT32) das2d_batch_tile<int16_t, float>(fr, a, t, lds);
@@ -561,7 +594,9 @@ __global__ __launch_bounds__(kThreads) void das2d_batch_kernel(const DasBatchArg
 template <typename RFT, typename OutT>
 static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool interp_tx, bool map,
                                hipStream_t stream) {
-    const uint64_t blocks = static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames;
+    const uint64_t units2d = static_cast<uint64_t>((a.n_depths + kWaveDepths - 1) / kWaveDepths) * a.n_rx * a.n_frames;
+    const uint64_t blocks = is3d ? static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames
+                                 : (units2d + kWavesPerGroup - 1) / kWavesPerGroup;
     dim3 grid(static_cast<uint32_t>(blocks)), block(kThreads);
 #define SUPRA_DAS_DISPATCH(KERNEL)                                                                                 \
     do {                                                                                                            \
