@@ -285,6 +285,7 @@ def bench_mv(args, torch):
             "plain_torch_ms": plain_rd_ms + plain_ms,
             "plain_torch_rawdelay_ms": plain_rd_ms,
             "speedup_vs_plain": (plain_rd_ms + plain_ms) / fused_ms,
+            "two_step_speedup_vs_plain": (plain_rd_ms + plain_ms) / (rd_ms + mv_ms),
         },
         "algorithmic_flop": flops,
         "achieved_TFLOPs": flops / (mv_ms * 1e-3) / 1e12,
