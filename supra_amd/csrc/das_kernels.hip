@@ -374,7 +374,11 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
         const float py = sc.dir_y() * p.d;
         p.py2 = py * py;
     }
-    const float d0 = load_uniform_float(a.depths, t.r0);
+    // Far-field test per wave, at the wave's first (smallest) depth: a wave
+    // whose 64 depths all see the saturated aperture takes the far path even
+    // when the workgroup's first wave does not.
+    const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
+    const float d0 = load_uniform_float(a.depths, min(t.r0 + wave * 64u, a.n_depths - 1u));
     const float ap0 = (((d0 * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
     const float aDT0 = ap0 * ap0;
     const float sq0 = sqrtf(aDT0);
