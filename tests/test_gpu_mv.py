@@ -207,3 +207,18 @@ def test_plain_torch_rawdelay_baseline(device):
     assert a.shape == b.shape
     assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-2
     rd.close()
+
+
+@pytest.mark.parametrize("n_rx,n_d,L,T", [(1, 1, 8, 5), (3, 2, 16, 5), (2, 11, 32, 5), (5, 65, 64, 3)])
+def test_mv_ragged_sizes(device, n_rx, n_d, L, T):
+    """Depth counts below, at and just past the temporal window and the depth
+    tile, one scanline, and the largest subarray: every sample matches the
+    FP64 oracle."""
+    rng = np.random.default_rng(n_d + L)
+    n_ch = max(L, 16) + 8
+    x = rng.normal(size=(n_rx, n_ch, n_d)).astype(np.float32) * 50
+    gpu = gpu_mv(x, sub_array_size=L, temporal_smoothing=T, loading_factor=0.05)
+    ref = O.mv(x, L, T, 1.5, False, 0.05)
+    assert gpu.shape == ref.shape == (n_d, n_rx)
+    err = rel_errors(gpu, ref)
+    assert np.quantile(err, 0.99) < 1e-4 and err.max() < 1e-2
