@@ -8,16 +8,20 @@
 // device-wide syncs per batch); here one launch covers the frame:
 //
 //   workgroup = (scanline, tile of Dt output depths), 256 lanes
-//   1. stage x[channel][tau] for the tile and its +-T halo in LDS (float), find
-//      each sample's aperture (first/last non-zero channel, :44-77), the
-//      subarray length L_t and the number of active subarrays (:79-113);
+//   1. stage x[channel][tau] for the tile and its +-T halo in LDS (float),
+//      read or, in the fused mode, computed from the RF; find each sample's
+//      aperture (first/last non-zero channel, :44-77, one wave and 64-channel
+//      ballots per sample), the subarray length L_t and the number of active
+//      subarrays (:79-113);
 //   2. R(tau) = sum_k (x_k - m)(x_k - m)^T for every halo sample, upper
-//      triangle packed in LDS; each lane owns 4x4 blocks and slides the two
-//      4-sample windows along k, so one LDS read feeds 8 FMAs (:151-195);
-//   3. one wave per output sample: R~ = sum over |tau - t| <= T (:197-228),
-//      diagonal loading alpha*trace/L_t (:239-305), then a Cholesky
-//      factorisation with row i of the matrix in lane i's registers, pivots
-//      and diagonal updates in FP64, triangular solves for w = R~^-1 1, an
+//      triangle packed in LDS, by diagonals: a K-term FP64 sum for the first
+//      entry of each diagonal, then the FP64 recurrence that adds the entering
+//      and removes the leaving product (:151-195); each lane walks a diagonal
+//      pair (d, L-1-d) so the trip counts match;
+//   3. one wave (L <= 64) or half-wave (L <= 32) per output sample: row i of
+//      R~ = sum over |tau - t| <= T (:197-228) in lane i's registers,
+//      diagonal loading alpha*trace/L_t (:239-305), a Cholesky factorisation
+//      with FP64 pivots and diagonal, triangular solves for w = R~^-1 1, an
 //      FP64 residual check with up to two steps of iterative refinement
 //      (FP32 factors, FP64 residual) for ill-conditioned samples, and the
 //      output y = sum_i (sum_k x[f+k+i]) w_i / (sum w * K^p) (:307-378).
