@@ -12,19 +12,23 @@
 // are correctly rounded, so the kernels reproduce the host restatement bit
 // for bit, including the summation order over the aperture.
 //
-// Launch layout (MI355X): one workgroup of 256 lanes (4 wave64) per
-// (frame, rx scanline, tile of 256 depths).  Lanes map to consecutive
-// depths, so the two RF gathers of one element hit a contiguous window of
-// ~2x256 samples of one channel.  Per-element quantities that do not depend
-// on depth are computed once per workgroup into LDS and broadcast.
+// Launch layout (MI355X).  2D: each wave64 is an independent unit of 64
+// consecutive depths of one (frame, rx scanline), with its own LDS tables
+// and no workgroup barrier (a 2080-deep image is 32.5 units, so almost no
+// lane idles in the last one).  3D: one workgroup of 256 lanes (4 wave64)
+// per (frame, rx scanline, tile of 256 depths), aperture elements staged in
+// LDS 128 at a time.  Lanes map to consecutive depths, so the two RF gathers
+// of one element hit a contiguous window of the channel.  Per-element
+// quantities that do not depend on depth are computed once per unit into
+// LDS and broadcast.
 //
 // The kernels are VALU-bound (a correctly rounded sqrt and a linear
 // interpolation per element and pixel), so the work per element is trimmed:
-//  * far-field tiles: once the F-number aperture aDT at the tile's first
-//    depth covers every aperture element and exceeds maxElementDistance
-//    (aDT grows monotonically with depth), membership is always true and the
-//    window argument no longer depends on depth, so each element's weight is
-//    looked up once per workgroup;
+//  * far field, decided per wave: once the F-number aperture aDT at the
+//    wave's first depth covers every aperture element and exceeds
+//    maxElementDistance (aDT grows monotonically with depth), membership is
+//    always true and the window argument no longer depends on depth, so each
+//    element's weight is looked up once when it is staged;
 //  * RF gathers are raw buffer loads (32-bit offsets, hardware range check
 //    returns 0 beyond the transmit's block), issued unconditionally so that
 //    the out-of-range cases of the reference become selects, not branches,
