@@ -45,6 +45,11 @@ class Batch:
                 raise ValueError("rf must be a contiguous device tensor [tx][ch][s]")
             if not (out.is_cuda and out.is_contiguous()):
                 raise ValueError("out must be a contiguous device tensor")
+            g = eng.geometry
+            if tuple(out.shape) != (g.num_depths, g.num_rx):
+                # the kernels write num_depths x num_rx samples through the bare pointer
+                raise ValueError(f"out must have shape {(g.num_depths, g.num_rx)} (the engine's image), "
+                                 f"got {tuple(out.shape)}")
             c.ctx = eng._ctx.value
             c.rf = rf.data_ptr()
             c.rf_dtype = _dtype_code(rf.dtype)

@@ -83,3 +83,16 @@ def test_batch_rejects_int16_mv_output(device):
                 out=torch.zeros((g.num_depths, g.num_rx), dtype=torch.int16, device="cuda"), fs_hz=40e6, mode=MV)
     with pytest.raises(N.SupraError):
         Batch([item], autotune=False)
+
+
+def test_batch_rejects_wrong_output_shape(device):
+    """The kernels write through the bare output pointer, so the Python
+    binding checks the image shape against the engine's geometry first."""
+    from supra_amd.batch import Batch, DAS
+    w = W.linear_acquisition(32, 300, 32, 40e6, 10, 1, 2)
+    g = w.geometry()
+    item = dict(engine=RxBeamformer(g), rf=torch.from_numpy(w.rf_host()).cuda(),
+                out=torch.zeros((g.num_depths - 1, g.num_rx), dtype=torch.float32, device="cuda"), fs_hz=40e6,
+                mode=DAS)
+    with pytest.raises(ValueError):
+        Batch([item], autotune=False)
