@@ -181,12 +181,19 @@ __global__ __launch_bounds__(256) void das2d_adjoint_kernel(const AdjointArgs a)
         const float dx = xe - x;
         const float* rf_line = a.rf + line;
         const bool want_dc = a.dc_part != nullptr;
-        for (uint32_t r = lane; r < a.n_depths; r += kWave) {
-            const float bt = beta[r];
-            const float d = depths[r];
+        // Lanes are consecutive depths, so their delays (never decreasing with
+        // depth) are sorted across the wave.  A lane whose upper sample ui+1
+        // is the next lane's lower sample hands its share to that lane, which
+        // adds it before its own LDS add: one LDS float atomic instead of two
+        // for that sample (the atomics' throughput bounds this kernel).
+        for (uint32_t r0 = 0; r0 < a.n_depths; r0 += kWave) {
+            const uint32_t r = r0 + lane;
+            const bool valid = r < a.n_depths;
+            const float bt = valid ? beta[r] : 0.0f;
+            const float d = valid ? depths[r] : 0.0f;
             // Same single-precision operations as the forward kernel.
             const float aDT = (((d * a.dt) * c) / (2.0f * a.F)) / c / a.dt;
-            if (bt == 0.0f || !(fabsf(dx) <= aDT)) continue;
+            const bool take = valid && bt != 0.0f && fabsf(dx) <= aDT;
             const float inv = 1.0f / fminf(aDT, med);
             const float w = s_lut[window_index(dx * inv)];
             const float px = x + dir_x * d;
@@ -199,13 +206,22 @@ __global__ __launch_bounds__(256) void das2d_adjoint_kernel(const AdjointArgs a)
             const float fr = tau - static_cast<float>(i);
             const uint32_t ui = static_cast<uint32_t>(i);
             const float bw = bt * w;
-            if (ui < a.n_s - 1u) {
-                atomicAdd(&s_line[ui], bw * (1.0f - fr));
-                atomicAdd(&s_line[ui + 1u], bw * fr);
-                if (want_dc) dc += (bw * (rf_line[ui + 1u] - rf_line[ui])) * geo;
-            } else if (ui == a.n_s - 1u && fr == 0.0f) {
-                atomicAdd(&s_line[ui], bw);
-            }
+            const bool inr = take && ui < a.n_s - 1u;
+            const bool edge = take && ui == a.n_s - 1u && fr == 0.0f;
+            const float lo = inr ? bw * (1.0f - fr) : (edge ? bw : 0.0f);
+            const float hi = inr ? bw * fr : 0.0f;
+            // link(L -> L+1) = inr(L) && valid(L+1) && ui(L+1) == ui(L) + 1, evaluated
+            // identically by both lanes
+            const uint32_t next_ui = __shfl_down(ui, 1, kWave);
+            const bool next_valid = __shfl_down(valid ? 1 : 0, 1, kWave) != 0 && lane + 1 < kWave;
+            const bool to_next = inr && next_valid && next_ui == ui + 1u;
+            const uint32_t prev_ui = __shfl_up(ui, 1, kWave);
+            const bool prev_inr = __shfl_up(inr ? 1 : 0, 1, kWave) != 0 && lane > 0;
+            const float prev_hi = __shfl_up(hi, 1, kWave);
+            const bool from_prev = prev_inr && valid && ui == prev_ui + 1u;
+            if (inr || edge || from_prev) atomicAdd(&s_line[ui], from_prev ? lo + prev_hi : lo);
+            if (inr && !to_next) atomicAdd(&s_line[ui + 1u], hi);
+            if (want_dc && inr) dc += (bw * (rf_line[ui + 1u] - rf_line[ui])) * geo;
         }
     }
     __syncthreads();
