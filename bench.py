@@ -37,6 +37,38 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu() -> dict:
+    """The host cores this process may run on (the CPU baseline uses all of
+    them), the machine's logical CPU count, its CPU model and the cgroup CPU
+    quota if one is set."""
+    info = {"threads": max(1, len(os.sched_getaffinity(0))), "nproc": os.cpu_count() or 1, "model": None,
+            "cgroup_cpu_quota": None}
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if quota != "max":
+            info["cgroup_cpu_quota"] = float(quota) / float(period)
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def normwise(a, ref) -> float:
+    """max |a - ref| / max |ref| over samples that saturate in neither (the
+    single-precision 0/0 of an element-aligned depth-0 pixel clamps to
+    -FLT_MAX; double precision does not see that element at depth 0)."""
+    a = np.asarray(a, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    m = (np.abs(ref) < 1e30) & (np.abs(a) < 1e30)
+    return float(np.abs(a[m] - ref[m]).max() / np.abs(ref[m]).max())
+
+
 def das_bytes(w, frames: int = 1) -> int:
     g = w.geometry()
     rf_bytes = w.num_tx * w.num_channels * w.num_samples * 2
@@ -132,8 +164,8 @@ def cpu_baseline_2d(w, g, rf, out, kw, budget_s: float):
     another until budget_s has passed), each checked against the GPU image."""
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    threads = max(1, min(threads, os.cpu_count() or 1))
+    cpu = host_cpu()
+    threads = cpu["threads"]
     rf_host = rf.cpu().numpy()
     out_host = out.cpu().numpy()
     done, exact, elapsed = 0, True, 0.0
@@ -143,10 +175,12 @@ def cpu_baseline_2d(w, g, rf, out, kw, budget_s: float):
         elapsed += time.perf_counter() - t0
         exact = exact and bool(np.array_equal(img, out_host[done]))
         done += 1
+    ref64 = O.das(g, rf_host[0], w.fs_hz, precision=64, threads=threads, **kw)
     return {"value": done * g.num_rx * g.num_depths / elapsed, "unit": "samples/s", "cores": threads,
             "kind": "port", "sample": f"{done} whole frames ({done * g.num_rx * g.num_depths} samples, "
                                       f"{elapsed:.1f} s)",
-            "gpu_matches_sample_bit_exact": exact}
+            "host": cpu, "gpu_matches_sample_bit_exact": exact,
+            "gpu_normwise_err_vs_fp64": normwise(out_host[0], ref64)}
 
 
 def bench_diff(args, torch, frames: int = 16):
@@ -197,8 +231,7 @@ def bench_diff(args, torch, frames: int = 16):
     log("[bench] timing the plain per-frame PyTorch form")
     p_wall, p_ev = time_launches(plain_step, 3, 1, torch)
     b = 2 * rf.numel() * 4 + 3 * frames * g.num_rx * g.num_depths * 4
-    dd.close()
-    return {
+    res = {
         "workload": f"linear_2d geometry, FP32 RF {frames}x128x64x2080, c ~ U[1450,1600] m/s, "
                     "loss ||DAS - DAS@1540||^2, dL/drf + dL/dc",
         "fwd_adj_ms": ev_ms,
@@ -208,6 +241,40 @@ def bench_diff(args, torch, frames: int = 16):
         "achieved_GBs": b / (ev_ms * 1e-3) / 1e9,
         "hbm_frac": b / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
+    if not args.no_cpu:
+        log("[bench] gradient errors against the FP64 host reverse mode")
+        res["gradient_errors"] = diff_errors(dd, rf, c, target, tmask, g, w, torch)
+    dd.close()
+    return res
+
+
+def diff_errors(dd, rf, c, target, tmask, g, w, torch):
+    """The step's dL/drf (normwise) and dL/dc (max relative over frames)
+    against the FP64 host reverse mode, on all pixels and with the pixels
+    where single and double precision take different discrete decisions
+    (window-table ties, membership, floor of the delay) left out of the loss."""
+    from oracle import oracle as O
+    from supra_amd import _native as N
+
+    threads = host_cpu()["threads"]
+    y, norm = dd.forward_raw(rf, c)
+    gi = torch.where(tmask & (y.abs() < 3e38), 2.0 * (y - target), 0.0)
+    c_np = c.cpu().numpy()
+    rf_host = rf.cpu().numpy()
+    flags = np.stack([O.das_decision_diff(g, w.num_samples, w.fs_hz, 1.0, cf * 1000.0, N.WINDOW_HANN,
+                                          threads=threads) for cf in c_np])
+    out = {"tie_pixels_per_frame": float((flags != 0).sum() / len(c_np)), "pixels_per_frame": int(flags[0].size)}
+    for label, keep in (("all_pixels", None), ("tie_pixels_out", flags == 0)):
+        gi_l = gi if keep is None else gi * torch.from_numpy(keep).to(gi.device)
+        g_rf, g_c = dd.adjoint_raw(rf, c, norm, gi_l)
+        r64, c64, _ = O.das_adjoint(g, rf_host, w.fs_hz, 1.0, c_np * 1000.0, N.WINDOW_HANN,
+                                    gi_l.double().cpu().numpy(), precision=64, threads=threads)
+        c64 = c64 * 1000.0
+        g_rf = g_rf.cpu().numpy()
+        g_c = g_c.cpu().numpy()
+        out[label] = {"dL_drf_normwise_vs_fp64": float(np.linalg.norm(g_rf - r64) / np.linalg.norm(r64)),
+                      "dL_dc_max_rel_vs_fp64": float((np.abs(g_c - c64) / np.abs(c64)).max())}
+    return out
 
 
 MV_L, MV_T, MV_P, MV_ALPHA = 32, 5, 1.5, 0.01
@@ -269,9 +336,14 @@ def bench_mv(args, torch):
     _, plain_rd_ms = time_launches(lambda: pdas.delayed(rf32, c_t, w.num_channels), 3, 1, torch)
     pixels = g.num_rx * g.num_depths
     flops = mv_flops_per_pixel(w.num_channels, MV_L, MV_T) * pixels
+    err = None
+    if not args.no_cpu:
+        log("[bench] minimum variance against the FP64 host oracle")
+        err = mv_errors(delayed.data, out, MV_L, MV_T, MV_P, MV_ALPHA, mv_step, torch)
     rd.close()
     engine.close()
     return {
+        "error_vs_fp64_oracle": err,
         "workload": "linear_2d frame -> RawDelay (F#1 Hann) -> MV L=32 T=5 loading trace/(100L), "
                     "128x2080 samples",
         "mv_pixels_per_s": pixels / (mv_ms * 1e-3),
@@ -291,6 +363,29 @@ def bench_mv(args, torch):
         "achieved_TFLOPs": flops / (mv_ms * 1e-3) / 1e12,
         "fp32_frac": flops / (mv_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
     }
+
+
+def mv_errors(delayed, out, L, T, p, alpha, mv_step, torch):
+    """Per-sample relative error of the MV image against the FP64 host oracle
+    (denominator floored at 1e-3 of the image maximum); samples whose
+    temporal window holds a -FLT_MAX pre-delayed value are 0 in single
+    precision, as in the reference, and are left out."""
+    from oracle import oracle as O
+
+    mv_step()
+    torch.cuda.synchronize()
+    x = delayed.cpu().numpy()
+    gpu = out.cpu().numpy().astype(np.float64)
+    ref = O.mv(x, L, T, p, False, alpha, threads=host_cpu()["threads"])
+    huge = (np.abs(x) > 1e19).any(axis=1)
+    tainted = np.zeros_like(huge)
+    for d in range(-T, T + 1):
+        tainted |= np.roll(huge, d, axis=1)
+    keep = ~tainted.T
+    den = np.maximum(np.abs(ref[keep]), 1e-3 * np.abs(ref[keep]).max())
+    e = np.abs(gpu[keep] - ref[keep]) / den
+    return {"samples": int(keep.sum()), "max": float(e.max()), "p99": float(np.quantile(e, 0.99)),
+            "median": float(np.median(e))}
 
 
 def bench_stream(args, torch, num: int = 48):
@@ -375,8 +470,8 @@ def cpu_baseline_3d(res, budget_s: float):
     from oracle import oracle as O
 
     w, g = res["workload"], res["geom"]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    threads = max(1, min(threads, os.cpu_count() or 1))
+    cpu = host_cpu()
+    threads = cpu["threads"]
     log(f"[bench] copying RF to host for the CPU baseline ({threads} threads)")
     rf_host = res["rf"].cpu().numpy()
     kw = w.das_kwargs()
@@ -391,12 +486,17 @@ def cpu_baseline_3d(res, budget_s: float):
         if dt >= budget_s or n_lines >= g.num_rx:
             break
         n_lines *= 2
-    gpu_vals = res["out"].cpu().numpy().ravel()[pix]
-    exact = bool(np.array_equal(gpu_vals, vals))
+    gpu_all = res["out"].cpu().numpy().ravel()
+    exact = bool(np.array_equal(gpu_all[pix], vals))
+    sub = np.sort(rng.choice(g.num_rx, size=min(64, g.num_rx), replace=False))
+    pix64 = (np.arange(g.num_depths)[:, None] * g.num_rx + sub[None, :]).ravel().astype(np.uint32)
+    ref64 = O.das(g, rf_host, w.fs_hz, pixels=pix64, precision=64, threads=threads, **kw)
     return {"value": pix.size / dt, "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": f"{lines.size} of {g.num_rx} scanlines x {g.num_depths} depths of the same volume "
                       f"({pix.size} samples, {dt:.1f} s)",
-            "gpu_matches_sample_bit_exact": exact}
+            "host": cpu, "gpu_matches_sample_bit_exact": exact,
+            "gpu_normwise_err_vs_fp64": normwise(gpu_all[pix64], ref64),
+            "fp64_sample": f"{sub.size} scanlines x {g.num_depths} depths"}
 
 
 def main() -> int:

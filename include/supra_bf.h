@@ -44,6 +44,7 @@ extern "C" {
 /* ---- element types (USRawData / USImage payloads) ---------------------- */
 #define SUPRA_INT16    1   /* int16_t: RF from the probe, or clamped image */
 #define SUPRA_FLOAT32  2   /* float */
+#define SUPRA_UINT8    3   /* uint8_t: log-compressed B-mode image */
 
 /* ---- WindowType (src/SupraLib/Beamformer/WindowFunction.h:37-44) ------- */
 #define SUPRA_WINDOW_RECTANGULAR 0
@@ -52,8 +53,10 @@ extern "C" {
 #define SUPRA_WINDOW_GAUSS       3
 
 /* ---- RxSampleBeamformer (src/SupraLib/Beamformer/RxBeamformerCuda.h:31-37) */
-#define SUPRA_BF_DELAY_AND_SUM   0
-#define SUPRA_BF_TEST_SIGNAL     3
+#define SUPRA_BF_DELAY_AND_SUM     0
+#define SUPRA_BF_DELAY_AND_STD_DEV 1   /* RxSampleBeamformerDelayAndStdDev */
+#define SUPRA_BF_COHERENCE_FACTOR  2   /* RxSampleBeamformerCoherenceFactorDelayAndSum */
+#define SUPRA_BF_TEST_SIGNAL       3
 
 /* Number of window-LUT entries (RxBeamformerCuda.h:78). */
 #define SUPRA_WINDOW_NUM_ENTRIES 64
@@ -156,7 +159,12 @@ void supra_bf_destroy(supra_bf_ctx* ctx);
  * (src/SupraLib/Beamformer/RxBeamformerCuda.cu:539-663; kernels :109-323).
  * rf:  device [num_tx][num_channels][num_samples] of rf_dtype
  * out: device [num_depths][num_rx_scanlines] of out_dtype (caller-owned)
- * sample_beamformer: SUPRA_BF_DELAY_AND_SUM or SUPRA_BF_TEST_SIGNAL.
+ * sample_beamformer: SUPRA_BF_DELAY_AND_SUM, SUPRA_BF_DELAY_AND_STD_DEV
+ * (RxSampleBeamformerDelayAndStdDev.h:36-270), SUPRA_BF_COHERENCE_FACTOR
+ * (RxSampleBeamformerCoherenceFactorDelayAndSum.h:33-258) or
+ * SUPRA_BF_TEST_SIGNAL.  In the two variants an aperture element whose delay
+ * is out of range contributes the value 0 (the reference reuses the previous
+ * element's value there).
  * Linear interpolation of the RF is always on, as in the reference
  * (RxBeamformerCuda.cu:598,623). */
 int supra_das_run(supra_bf_ctx* ctx, int32_t sample_beamformer,
@@ -221,11 +229,15 @@ int supra_das_adjoint(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf,
                       float* grad_rf, double* grad_c, void* stream);
 
 /* RawDelay::performDelay<ChannelDataType, OutputType>
- * (src/SupraLib/Beamformer/RawDelay.cu:555-650; kernel :404-468, per-sample
- * math :221-312).  2D geometries.  For each rx scanline's highest-weight
- * transmit, writes every aperture element's delayed, apodised sample times
- * n/W: out[scanline][j][depth] (device, out_dtype), j = element - first;
- * channels j >= last - first are written as 0.  out_sampling_frequency_hz
+ * (src/SupraLib/Beamformer/RawDelay.cu:555-650; 2D kernel :404-468, per-sample
+ * math :221-312; 3D kernel :315-402, per-sample math :126-220).  For each rx
+ * scanline's highest-weight transmit, writes every aperture element's
+ * delayed, apodised sample times n/W: out[scanline][j][depth] (device,
+ * out_dtype).  2D: j = element - first; channels j >= last - first are
+ * written as 0.  3D: j = (ex - first.x) + (ey - first.y) * element_layout[0]
+ * (the largest j must be below num_channels); slots no aperture element maps
+ * to are written as 0, and so are elements outside the F-number aperture or
+ * with their delay out of range.  out_sampling_frequency_hz
  * (may be NULL) receives the reference's output rate 1/(depth/c/num_depths)
  * (RawDelay.cu:642). */
 int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dtype,
@@ -311,6 +323,83 @@ int  supra_batch_launch(supra_batch* batch, void* stream);
 int  supra_batch_latencies(supra_batch* batch, double* frame_us, double* total_us);
 int  supra_batch_config(const supra_batch* batch, char* buffer, size_t buffer_size);
 void supra_batch_destroy(supra_batch* batch);
+
+/* ---- raw-data mock files ---------------------------------------------------
+ * SUPRA's recorded-acquisition format: metadata in ".json"
+ * (RxBeamformerParameters::writeMetaDataForMock / readMetaDataForMockJson,
+ * src/SupraLib/Beamformer/RxBeamformerParameters.cpp:93-156, :311-450) or in
+ * the ASCII ".mock" v1 format (readMetaDataForMockAscii, :187-309; scanline
+ * record of USImageProperties.cpp:59-79), and the frames as consecutive int16
+ * [tx][channel][sample] blocks of one file (UltrasoundInterfaceRawDataMock.cpp:
+ * 49-92, 187-204).  Host memory only. */
+typedef struct supra_mock_meta {
+    uint32_t num_elements;
+    uint32_t element_layout[2];
+    uint32_t num_channels;              /* numReceivedChannels */
+    uint32_t num_samples;
+    uint32_t num_tx;                    /* numTxScanlines */
+    uint32_t rx_scanline_layout[2];
+    uint32_t num_depths;                /* rxNumDepths */
+    double   depth;                     /* [mm] */
+    double   sampling_frequency_hz;
+    double   speed_of_sound_mm_per_s;
+} supra_mock_meta;
+
+/* Sizes and scalars of a .json or .mock metadata file (by suffix). */
+int supra_mock_read_meta(const char* path, supra_mock_meta* meta);
+/* The receive geometry of the file into caller arrays sized from
+ * supra_mock_read_meta: scanlines[rx_scanline_layout[0] * [1]] (x fastest),
+ * depths[num_depths] (.json: depth * i / (num_depths - 1), as the reference
+ * deduces them), element_x/element_y[num_elements]. */
+int supra_mock_read_geometry(const char* path, supra_scanline_rx* scanlines, float* depths,
+                             float* element_x, float* element_y);
+/* writeMetaDataForMock: the .json metadata (doubles with 17 significant
+ * digits, so a read gives back the same values). */
+int supra_mock_write_json(const char* path, const supra_mock_meta* meta, const supra_rx_geometry* geometry);
+/* The ASCII .mock v1 layout that readMetaDataForMockAscii reads (doubles with
+ * 17 significant digits where the reference's writer used 9). */
+int supra_mock_write_ascii(const char* path, const supra_mock_meta* meta, const supra_rx_geometry* geometry);
+/* Frames in a raw file: file size / (num_tx * num_channels * num_samples * 2). */
+int supra_mock_count_frames(const char* raw_path, const supra_mock_meta* meta, uint64_t* num_frames);
+/* Frames first .. first + num_frames - 1 into host memory out
+ * [frame][tx][channel][sample]. */
+int supra_mock_read_frames(const char* raw_path, const supra_mock_meta* meta, uint64_t first_frame,
+                           uint32_t num_frames, int16_t* out);
+
+/* ---- after the beamformer: envelope detection and log compression ------- */
+
+/* HilbertFirEnvelope::demodulate<InputType, OutputType>
+ * (src/SupraLib/Beamformer/HilbertFirEnvelope.cu:28-104; filter of
+ * FirFilterFactory.h:55-200 with FilterTypeHilbertTransformer and
+ * FilterWindowHamming; HilbertFirEnvelopeNode.cpp:42: filterLength 1..257,
+ * default 65).  image/out: device [num_samples][num_scanlines], scanline
+ * fastest (a USImage).  in_dtype SUPRA_INT16 or SUPRA_FLOAT32, out_dtype
+ * SUPRA_INT16 or SUPRA_FLOAT32.  out = clampCast(sqrt(x^2 + (h * x)^2)), the
+ * FIR over samples t - N/2 .. t + N/2 inside the image.  For an even N the
+ * reference reads one coefficient past its filter; here that tap adds
+ * nothing. */
+int supra_envelope_run(const void* image, int32_t in_dtype, uint32_t num_scanlines, uint32_t num_samples,
+                       uint32_t filter_length, void* out, int32_t out_dtype, void* stream);
+
+/* LogCompressor::compress<InputType, OutputType>
+ * (src/SupraLib/Beamformer/LogCompressor.cu:20-64; LogCompressorNode.cpp:42-45:
+ * dynamicRange 1..200 dB (80), gain 0.01..2 (1), inMax 10..50000 (1024)).
+ * count values of in_dtype (SUPRA_UINT8, SUPRA_INT16, SUPRA_FLOAT32) to
+ * out_dtype (SUPRA_UINT8 or SUPRA_FLOAT32):
+ * clampCast(log10(|x| * 10^(dB/20) / inMax + 1) * gain * 255 / log10(10^(dB/20) + 1)),
+ * single precision, inMax converted to the input element type first (as the
+ * reference's functor argument). */
+int supra_logcompress_run(const void* in, int32_t in_dtype, uint64_t count, double dynamic_range_db,
+                          double gain, double in_max, void* out, int32_t out_dtype, void* stream);
+
+/* supra_envelope_run (float envelope) followed by supra_logcompress_run (float
+ * input) in one launch: HilbertFirEnvelopeNode -> LogCompressorNode without
+ * the intermediate image in memory.  No reference counterpart (the reference
+ * chains the two nodes through a device buffer).  out_dtype SUPRA_UINT8 or
+ * SUPRA_FLOAT32; the result is bit-identical to the two calls. */
+int supra_envelope_logcompress_run(const void* image, int32_t in_dtype, uint32_t num_scanlines,
+                                   uint32_t num_samples, uint32_t filter_length, double dynamic_range_db,
+                                   double gain, double in_max, void* out, int32_t out_dtype, void* stream);
 
 /* Window LUT of the reference (WindowFunction.cpp:18-47, WindowFunction.h:122-138),
  * computed on the host: SUPRA_WINDOW_NUM_ENTRIES floats, max-normalised. */

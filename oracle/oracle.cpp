@@ -190,7 +190,7 @@ struct DasCall {
     float wparam;
     int interp_tx;
     int32_t offset;
-    int testsig;
+    int kind;        // sample beamformer: SUPRA_BF_DELAY_AND_SUM, _DELAY_AND_STD_DEV, _COHERENCE_FACTOR, _TEST_SIGNAL
     int out_dtype;
 };
 
@@ -241,7 +241,7 @@ T das_pixel_2d(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32
         if (!use) continue;
         const supra_tx_params& tp = sc.tx[k];
         T s_local;
-        if (call.testsig) {
+        if (call.kind == SUPRA_BF_TEST_SIGNAL) {
             s_local = test_signal<T>(false, x, T(0), dirx, diry, dirz, d, dt, c);
         } else {
             const T t0 = G.t0[4 * s + k];
@@ -278,6 +278,51 @@ T das_pixel_2d(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32
                 }
             }
             s_local = (nadd > 0) ? (sample / wsum) * static_cast<T>(nadd) : T(0);
+            if (call.kind == SUPRA_BF_DELAY_AND_STD_DEV || call.kind == SUPRA_BF_COHERENCE_FACTOR) {
+                // RxSampleBeamformerDelayAndStdDev.h:161-270 /
+                // RxSampleBeamformerCoherenceFactorDelayAndSum.h:152-258: a second
+                // pass over the aperture with the DAS value as `mean` / `resultDas`.
+                // An element whose delay is out of range has the value 0 (the
+                // reference keeps the previous element's value there).
+                const T das_v = s_local;
+                T sd = 0, cs = 0, te = 0;
+                for (int32_t e = tp.first_active_element[0]; e < tp.last_active_element[0]; e++) {
+                    uint32_t ch;
+                    if (g.element_to_channel) {
+                        if (g.element_to_channel[e] < 0) continue;
+                        ch = static_cast<uint32_t>(g.element_to_channel[e]);
+                    } else {
+                        ch = static_cast<uint32_t>(e) % call.rf.n_ch;
+                    }
+                    const T xe = G.ex[e];
+                    if (!(std::fabs(xe - x) <= aDT)) continue;
+                    const T w = window_get(lut, (xe - x) * inv);
+                    const T a = xe - (x + dirx * d);
+                    const T b = diry * d;
+                    T tau = t0 + (vsqrt(a * a + b * b) + d);
+                    tau = tau + off;
+                    const int32_t i = static_cast<int32_t>(vfloor(tau));
+                    const T f = tau - static_cast<T>(i);
+                    const uint64_t base = tx_base + static_cast<uint64_t>(ch) * call.rf.n_s;
+                    T interp = 0;
+                    if (static_cast<uint32_t>(i) < call.rf.n_s - 1)
+                        interp = (T(1) - f) * rf_at<T>(call.rf, base + static_cast<uint32_t>(i)) +
+                                 f * rf_at<T>(call.rf, base + static_cast<uint32_t>(i) + 1);
+                    else if (static_cast<uint32_t>(i) < call.rf.n_s && f == T(0))
+                        interp = rf_at<T>(call.rf, base + static_cast<uint32_t>(i));
+                    if (call.kind == SUPRA_BF_DELAY_AND_STD_DEV) {
+                        const T value = w * interp;
+                        sd += (value - das_v) * (value - das_v);
+                    } else {
+                        cs += interp;
+                        te += interp * interp;
+                    }
+                }
+                if (call.kind == SUPRA_BF_DELAY_AND_STD_DEV)
+                    s_local = (nadd > 0) ? vsqrt(sd) / wsum * static_cast<T>(nadd) : T(0);
+                else
+                    s_local = (nadd > 0) ? cs * cs / te / static_cast<T>(nadd) * das_v : T(0);
+            }
         }
         if (call.interp_tx) s_interp += static_cast<T>(static_cast<float>(sc.tx_weights[k])) * s_local;
         else s_interp += s_local;
@@ -311,7 +356,7 @@ T das_pixel_3d(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32
         if (!use) continue;
         const supra_tx_params& tp = sc.tx[k];
         T s_local;
-        if (call.testsig) {
+        if (call.kind == SUPRA_BF_TEST_SIGNAL) {
             s_local = test_signal<T>(true, x, z, dirx, diry, dirz, d, dt, c);
         } else {
             const T t0 = G.t0[4 * s + k];
@@ -358,6 +403,53 @@ T das_pixel_3d(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32
                 }
             }
             s_local = (nadd > 0) ? (sample / wsum) * static_cast<T>(nadd) : T(0);
+            if (call.kind == SUPRA_BF_DELAY_AND_STD_DEV || call.kind == SUPRA_BF_COHERENCE_FACTOR) {
+                // RxSampleBeamformerDelayAndStdDev.h:36-158 /
+                // RxSampleBeamformerCoherenceFactorDelayAndSum.h:33-150 (3D), as in 2D.
+                const T das_v = s_local;
+                T sd = 0, cs = 0, te = 0;
+                for (uint32_t ex = tp.first_active_element[0]; ex < tp.last_active_element[0]; ex++) {
+                    for (uint32_t ey = tp.first_active_element[1]; ey < tp.last_active_element[1]; ey++) {
+                        const uint32_t e = ex + ey * lx;
+                        uint32_t ch;
+                        if (g.element_to_channel) {
+                            if (g.element_to_channel[e] < 0) continue;
+                            ch = static_cast<uint32_t>(g.element_to_channel[e]);
+                        } else {
+                            ch = e % call.rf.n_ch;
+                        }
+                        const T xe = G.ex[e], ze = G.ey[e];
+                        const T dxs = xe - x, dzs = ze - z;
+                        if (!((dxs * dxs + dzs * dzs) <= aDT)) continue;
+                        const T w = vsqrt(window_get(lut, dxs * invx) * window_get(lut, dzs * invz));
+                        const T ax = xe - (x + dirx * d);
+                        const T az = ze - (z + dirz * d);
+                        const T ay = diry * d;
+                        T tau = t0 + (vsqrt((ax * ax + az * az) + ay * ay) + d);
+                        tau = tau + off;
+                        const T fl = vfloor(tau);
+                        const uint32_t i = (fl > T(0)) ? static_cast<uint32_t>(fl) : 0u;
+                        const T f = tau - static_cast<T>(i);
+                        const uint64_t base = tx_base + static_cast<uint64_t>(ch) * call.rf.n_s;
+                        T interp = 0;
+                        if (i < call.rf.n_s - 1)
+                            interp = (T(1) - f) * rf_at<T>(call.rf, base + i) + f * rf_at<T>(call.rf, base + i + 1);
+                        else if (i < call.rf.n_s && f == T(0))
+                            interp = rf_at<T>(call.rf, base + i);
+                        if (call.kind == SUPRA_BF_DELAY_AND_STD_DEV) {
+                            const T value = w * interp;
+                            sd += (value - das_v) * (value - das_v);
+                        } else {
+                            cs += interp;
+                            te += interp * interp;
+                        }
+                    }
+                }
+                if (call.kind == SUPRA_BF_DELAY_AND_STD_DEV)
+                    s_local = (nadd > 0) ? vsqrt(sd) / wsum * static_cast<T>(nadd) : T(0);
+                else
+                    s_local = (nadd > 0) ? cs * cs / te / static_cast<T>(nadd) * das_v : T(0);
+            }
         }
         if (call.interp_tx) s_interp += static_cast<T>(static_cast<float>(sc.tx_weights[k])) * s_local;
         else s_interp += s_local;
@@ -442,16 +534,97 @@ void rawdelay_pixel(const DasCall& call, const DtGeometry<T>& G, const T* lut, u
     }
 }
 
+// One pixel of the 3D pre-delay: rxDelayDTSPACE3DKernel + sampleDelay3D
+// (src/SupraLib/Beamformer/RawDelay.cu:126-220, 315-402).  Highest-weight
+// transmit only, no additional offset; aperture radius squared, separable
+// window sqrt(w_x w_z) and the saturating floor of the 3D DAS; writes
+// w*interp*(n/W) of aperture element (ex, ey) into slot
+// (ex - first.x) + (ey - first.y) * element_layout.x of scanline s.  Slots no
+// aperture element maps to are 0, and so is an element outside the F-number
+// aperture or with its delay out of range: the reference's 3D loop keeps
+// `sample` from the previous element there (it is declared outside the
+// element loops, RawDelay.cu:150, while the 2D loop resets it per element,
+// :264), which is not reproduced.
+// Pinned by: tests/test_oracle_mv.py::test_rawdelay3d_sums_to_das3d — the
+// channel sum of the pre-delayed volume is the 3D DAS volume (same F-number
+// and window, no offset), and a constant RF with a rectangular window gives
+// k in every in-aperture slot.
+template <typename T>
+void rawdelay_pixel_3d(const DasCall& call, const DtGeometry<T>& G, const T* lut, uint32_t s, uint32_t r, T* out) {
+    const supra_rx_geometry& g = *call.g;
+    const supra_scanline_rx& sc = g.scanlines[s];
+    const uint32_t n_ch = call.rf.n_ch, n_d = g.num_depths, lx = g.element_layout[0];
+    T* dst = out + static_cast<uint64_t>(s) * n_ch * n_d + r;
+    for (uint32_t j = 0; j < n_ch; j++) dst[static_cast<uint64_t>(j) * n_d] = 0;
+    const T dt = static_cast<T>(1.0 / call.fs), c = static_cast<T>(call.c), F = static_cast<T>(call.F);
+    const T d = G.depth[r];
+    const T t_ap = (((d * dt) * c) / (T(2) * F)) / c / dt;
+    const T aDT = t_ap * t_ap;
+    const T x = G.sx[s], z = G.sz[s], dirx = G.dirx[s], diry = G.diry[s], dirz = G.dirz[s];
+    const T sq = vsqrt(aDT);
+    const T invx = T(1) / vmin(sq, G.medx[s]);
+    const T invz = T(1) / vmin(sq, G.medz[s]);
+    const int k = select_tx(sc);
+    if (!(sc.tx_weights[k] > 0.0)) return;
+    const supra_tx_params& tp = sc.tx[k];
+    const T t0 = G.t0[4 * s + k];
+    auto weight = [&](uint32_t e, bool& member) {
+        const T dxs = G.ex[e] - x, dzs = G.ey[e] - z;
+        member = (dxs * dxs + dzs * dzs) <= aDT;
+        return member ? vsqrt(window_get(lut, dxs * invx) * window_get(lut, dzs * invz)) : T(0);
+    };
+    T wsum = 0;
+    int nadd = 0;
+    for (uint32_t ex = tp.first_active_element[0]; ex < tp.last_active_element[0]; ex++)
+        for (uint32_t ey = tp.first_active_element[1]; ey < tp.last_active_element[1]; ey++) {
+            bool m;
+            const T w = weight(ex + ey * lx, m);
+            if (m) {
+                wsum += w;
+                nadd++;
+            }
+        }
+    T scale = T(1);
+    if (nadd > 0) scale = T(1) / wsum * static_cast<T>(nadd);
+    const uint64_t tx_base = static_cast<uint64_t>(tp.tx_scanline_idx) * n_ch * call.rf.n_s;
+    for (uint32_t ex = tp.first_active_element[0]; ex < tp.last_active_element[0]; ex++)
+        for (uint32_t ey = tp.first_active_element[1]; ey < tp.last_active_element[1]; ey++) {
+            const uint32_t e = ex + ey * lx;
+            const uint32_t slot = (ex - tp.first_active_element[0]) + (ey - tp.first_active_element[1]) * lx;
+            if (slot >= n_ch) continue;
+            bool m;
+            const T w = weight(e, m);
+            T sample = 0;
+            if (m) {
+                const T ax = G.ex[e] - (x + dirx * d);
+                const T az = G.ey[e] - (z + dirz * d);
+                const T ay = diry * d;
+                const T tau = t0 + (vsqrt((ax * ax + az * az) + ay * ay) + d);
+                const T fl = vfloor(tau);
+                const uint32_t i = (fl > T(0)) ? static_cast<uint32_t>(fl) : 0u;
+                const T f = tau - static_cast<T>(i);
+                const uint64_t base = tx_base + static_cast<uint64_t>(e % n_ch) * call.rf.n_s;
+                if (i < call.rf.n_s - 1)
+                    sample = w * ((T(1) - f) * rf_at<T>(call.rf, base + i) + f * rf_at<T>(call.rf, base + i + 1));
+                else if (i < call.rf.n_s && f == T(0))
+                    sample = w * rf_at<T>(call.rf, base + i);
+            }
+            dst[static_cast<uint64_t>(slot) * n_d] = clamp_out(sample * scale, call.out_dtype);
+        }
+}
+
 template <typename T>
 void rawdelay_run(const DasCall& call, T* out, int nthreads) {
     const supra_rx_geometry& g = *call.g;
     DtGeometry<T> G = to_dt_space<T>(g, 1.0 / call.fs, call.c);
     T lut[kEntries];
     window_lut<T>(call.window, call.wparam, lut);
+    const bool is3d = g.rx_scanline_layout[0] > 1 && g.rx_scanline_layout[1] > 1;
     const uint64_t total = static_cast<uint64_t>(g.num_rx_scanlines) * g.num_depths;
     parallel_for(static_cast<int64_t>(total), nthreads, [&](int64_t q) {
         const uint32_t s = static_cast<uint32_t>(q / g.num_depths), r = static_cast<uint32_t>(q % g.num_depths);
-        rawdelay_pixel<T>(call, G, lut, s, r, out);
+        if (is3d) rawdelay_pixel_3d<T>(call, G, lut, s, r, out);
+        else rawdelay_pixel<T>(call, G, lut, s, r, out);
     });
 }
 
@@ -747,6 +920,163 @@ void das_adjoint_frame(const DasCall& call, const double* grad_image, T* grad_rf
     *grad_c = -dc / call.c;
 }
 
+// Pixels of the 2D DAS where two evaluations take different discrete
+// decisions for some aperture element (no reference counterpart: a
+// classifier for the accuracy analysis of the adjoint).  Evaluation A uses
+// speed of sound c_a in precision prec_a (32 or 64), B c_b in prec_b.  The
+// decisions are those of das_pixel_2d: aperture membership
+// |x_e - x| <= aDT, the window value picked by the rounded table index
+// (flagged only where the two indices select different values of the
+// single-precision table), the floor of the delay, and the range test on it.
+// Flags: 1 membership, 2 window value, 4 floor, 8 range.  Membership and
+// window are ratios of dt-space lengths (independent of c in exact
+// arithmetic), floor and range are not.
+// Pinned by: tests/test_oracle_adjoint.py::test_decision_diff_classifier — a
+// rectangular window never sets flag 2, outside the FP32-vs-FP64 flags the
+// two images agree to single-precision rounding of the delays and inside them
+// the large differences sit, and equal evaluations flag nothing.
+struct DecisionEval {
+    double c;
+    int precision;
+};
+
+void das_decision_diff(const DasCall& call, DecisionEval A, DecisionEval B, uint8_t* flags, int nthreads) {
+    const supra_rx_geometry& g = *call.g;
+    const double dt_d = 1.0 / call.fs;
+    const DtGeometry<float> Gfa = to_dt_space<float>(g, dt_d, A.c), Gfb = to_dt_space<float>(g, dt_d, B.c);
+    const DtGeometry<double> Gda = to_dt_space<double>(g, dt_d, A.c), Gdb = to_dt_space<double>(g, dt_d, B.c);
+    float lutf[kEntries];
+    window_lut<float>(call.window, call.wparam, lutf);
+    const uint32_t n_rx = g.num_rx_scanlines, n_d = g.num_depths, n_s = call.rf.n_s;
+    parallel_for(static_cast<int64_t>(n_rx), nthreads, [&](int64_t si) {
+        const uint32_t s = static_cast<uint32_t>(si);
+        const supra_scanline_rx& sc = g.scanlines[s];
+        const int sel = select_tx(sc);
+        for (uint32_t r = 0; r < n_d; r++) {
+            uint8_t fl = 0;
+            auto eval = [&](auto one, const auto& G, double c_d, int k, int32_t e, bool& member, int& widx,
+                            int32_t& fi, bool& inr) {
+                using T = decltype(one);
+                const T dt = static_cast<T>(dt_d), c = static_cast<T>(c_d), F = static_cast<T>(call.F);
+                const T d = G.depth[r];
+                const T aDT = (((d * dt) * c) / (T(2) * F)) / c / dt;
+                const T inv = T(1) / vmin(aDT, G.medx[s]);
+                const T xe = G.ex[e], x = G.sx[s];
+                member = std::fabs(xe - x) <= aDT;
+                widx = static_cast<int>(window_index_of<T>((xe - x) * inv));
+                const T a = xe - (x + G.dirx[s] * d), b = G.diry[s] * d;
+                T tau = G.t0[4 * s + k] + (vsqrt(a * a + b * b) + d);
+                tau = tau + static_cast<T>(call.offset);
+                fi = static_cast<int32_t>(vfloor(tau));
+                const T f = tau - static_cast<T>(fi);
+                inr = static_cast<uint32_t>(fi) < n_s - 1 || (static_cast<uint32_t>(fi) < n_s && f == T(0));
+            };
+            auto eval_at = [&](const DecisionEval& E, bool is_a, int k, int32_t e, bool& m, int& w, int32_t& f,
+                               bool& in) {
+                if (E.precision == 64) eval(0.0, is_a ? Gda : Gdb, E.c, k, e, m, w, f, in);
+                else eval(0.0f, is_a ? Gfa : Gfb, E.c, k, e, m, w, f, in);
+            };
+            for (int k = 0; k < 4; k++) {
+                const bool use = call.interp_tx ? (sc.tx_weights[k] > 0.0) : (k == sel && sc.tx_weights[k] > 0.0);
+                if (!use) continue;
+                const supra_tx_params& tp = sc.tx[k];
+                for (int32_t e = tp.first_active_element[0]; e < tp.last_active_element[0]; e++) {
+                    if (g.element_to_channel && g.element_to_channel[e] < 0) continue;
+                    bool ma, mb, ra, rb;
+                    int wa, wb;
+                    int32_t fa, fb;
+                    eval_at(A, true, k, e, ma, wa, fa, ra);
+                    eval_at(B, false, k, e, mb, wb, fb, rb);
+                    if (ma != mb) { fl |= 1; continue; }
+                    if (!ma) continue;
+                    if (wa != wb && lutf[wa] != lutf[wb]) fl |= 2;
+                    if (fa != fb) fl |= 4;
+                    if (ra != rb) fl |= 8;
+                }
+            }
+            flags[static_cast<uint64_t>(r) * n_rx + s] = fl;
+        }
+    });
+}
+
+// ---- envelope detection and log compression (after beamforming) -----------
+
+// FirFilterFactory::createFilter(length, FilterTypeHilbertTransformer,
+// FilterWindowHamming) (src/SupraLib/utilities/FirFilterFactory.h:55-200):
+// h[n] = 2/(pi k) for odd k = n - (N-1)/2, else 0, evaluated in double and
+// cast, then multiplied by the Hamming window 0.54 - 0.46 cos(2 pi n/(N-1))
+// cast to the element type.
+// Pinned by: tests/test_oracle_postprocess.py — the closed form in numpy
+// float64, antisymmetry h[c-k] = -h[c+k], zero even taps.
+template <typename T>
+void hilbert_filter(uint32_t n, T* h) {
+    const int half = (static_cast<int>(n) - 1) / 2;
+    const size_t max_n = n - 1;
+    for (uint32_t i = 0; i < n; i++) {
+        const int k = static_cast<int>(i) - half;
+        h[i] = (k % 2 == 0) ? static_cast<T>(0) : static_cast<T>(2.0 / (M_PI * k));
+        h[i] *= static_cast<T>(0.54 - 0.46 * std::cos(2 * M_PI * i / max_n));
+    }
+}
+
+// kernelFilterDemodulation (src/SupraLib/Beamformer/HilbertFirEnvelope.cu:28-59):
+// for sample t of scanline s of an image [sample][scanline], the FIR over
+// samples t - N/2 .. t + N/2 inside the image (tap j = t' - (t - N/2)), then
+// sqrt(x^2 + acc^2).  Taps past the end of an even-length filter (the
+// reference reads one element beyond its filter there) contribute nothing.
+// Pinned by: tests/test_oracle_postprocess.py — a pure tone A cos(w t) in
+// the filter's passband has the envelope A away from the image ends (the
+// analytic signal), a constant image has the envelope |c| (the Hilbert
+// transform of a constant is 0 up to the truncated filter's DC leak).
+template <typename T>
+void envelope_run(const void* in, int in_dtype, uint32_t n_rx, uint32_t n_s, uint32_t n_taps, T* out, int nthreads) {
+    std::vector<T> h(n_taps);
+    hilbert_filter<T>(n_taps, h.data());
+    auto at = [&](uint64_t i) -> T {
+        return in_dtype == SUPRA_INT16 ? static_cast<T>(static_cast<const int16_t*>(in)[i])
+                                       : static_cast<T>(static_cast<const float*>(in)[i]);
+    };
+    const int half = static_cast<int>(n_taps) / 2;
+    parallel_for(static_cast<int64_t>(n_rx) * n_s, nthreads, [&](int64_t q) {
+        const uint32_t t = static_cast<uint32_t>(q / n_rx), s = static_cast<uint32_t>(q % n_rx);
+        T acc = 0;
+        int j = 0;
+        for (int c = static_cast<int>(t) - half; c <= static_cast<int>(t) + half; c++, j++)
+            if (c >= 0 && c < static_cast<int>(n_s) && j < static_cast<int>(n_taps))
+                acc += at(static_cast<uint64_t>(c) * n_rx + s) * h[j];
+        const T v = at(static_cast<uint64_t>(t) * n_rx + s);
+        out[q] = vsqrt(v * v + acc * acc);
+    });
+}
+
+// thrustLogcompress (src/SupraLib/Beamformer/LogCompressor.cu:20-64):
+// y = log10(|x| * inScale + 1) * scaleOverDenominator with
+// inScale = (float)(10^(dB/20) / inMax) and scaleOverDenominator =
+// (float)(gain * outMax / log10(10^(dB/20) + 1)); inMax is first converted
+// to the input element type (as the reference's functor argument), outMax is
+// 255 for float and uint8 outputs.  T = float evaluates the reference's
+// single-precision arithmetic, T = double the intended math.
+// Pinned by: tests/test_oracle_postprocess.py — y(0) = 0, y(inMax) = gain *
+// 255, and the closed form in numpy float64 on random inputs.
+template <typename T>
+void logcompress_run(const void* in, int in_dtype, uint64_t count, double dynamic_range_db, double gain,
+                     double in_max, T* out, int nthreads) {
+    const double dr = std::pow(10.0, dynamic_range_db / 20);
+    double in_max_t = in_max;
+    if (in_dtype == SUPRA_INT16) in_max_t = static_cast<int16_t>(in_max);
+    else if (in_dtype == SUPRA_UINT8) in_max_t = static_cast<uint8_t>(in_max);
+    else in_max_t = static_cast<float>(in_max);
+    const T in_scale = static_cast<T>(dr / in_max_t);
+    const T scale_den = static_cast<T>(gain * 255.0 / std::log10(dr + 1));
+    parallel_for(static_cast<int64_t>(count), nthreads, [&](int64_t i) {
+        T x;
+        if (in_dtype == SUPRA_INT16) x = static_cast<T>(static_cast<const int16_t*>(in)[i]);
+        else if (in_dtype == SUPRA_UINT8) x = static_cast<T>(static_cast<const uint8_t*>(in)[i]);
+        else x = static_cast<T>(static_cast<const float*>(in)[i]);
+        out[i] = std::log10(std::fabs(x) * in_scale + T(1)) * scale_den;
+    }, 4096);
+}
+
 }  // namespace
 
 extern "C" {
@@ -761,34 +1091,37 @@ int oracle_window_lut(int window_type, float window_parameter, int precision, vo
 
 // DAS over all pixels ([depth][scanline]) or the listed pixel indices
 // (r * num_rx + s).  out: float (precision 32) or double (precision 64).
+// sample_beamformer: SUPRA_BF_DELAY_AND_SUM, _DELAY_AND_STD_DEV,
+// _COHERENCE_FACTOR or _TEST_SIGNAL.
 int oracle_das(const supra_rx_geometry* g, const void* rf, int rf_dtype, uint32_t n_tx, uint32_t n_ch, uint32_t n_s,
                double fs, double f_number, double c_mm_per_s, int window_type, float window_parameter,
-               int interpolate_transmits, int32_t additional_offset, int test_signal, int out_dtype, int precision,
+               int interpolate_transmits, int32_t additional_offset, int sample_beamformer, int out_dtype, int precision,
                const uint32_t* pixels, uint64_t n_pixels, void* out, int nthreads) {
     if (!g || !rf || !out || n_tx == 0 || n_ch == 0 || n_s == 0) return -1;
+    if (sample_beamformer < SUPRA_BF_DELAY_AND_SUM || sample_beamformer > SUPRA_BF_TEST_SIGNAL) return -1;
     for (uint32_t i = 0; i < g->num_rx_scanlines; i++)
         for (int k = 0; k < 4; k++)
             if (g->scanlines[i].tx_weights[k] > 0.0 && g->scanlines[i].tx[k].tx_scanline_idx >= n_tx) return -2;
     DasCall call{g, RfView{rf, rf_dtype, n_tx, n_ch, n_s}, fs, f_number, c_mm_per_s, window_type, window_parameter,
-                 interpolate_transmits, additional_offset, test_signal, out_dtype};
+                 interpolate_transmits, additional_offset, sample_beamformer, out_dtype};
     if (precision == 64) das_run<double>(call, pixels, n_pixels, static_cast<double*>(out), nthreads);
     else das_run<float>(call, pixels, n_pixels, static_cast<float*>(out), nthreads);
     return 0;
 }
 
 
-// Pre-delayed RF out[scanline][channel][depth] (2D geometries), float
+// Pre-delayed RF out[scanline][channel][depth] (2D and 3D geometries), float
 // (precision 32) or double (precision 64).
 int oracle_rawdelay(const supra_rx_geometry* g, const void* rf, int rf_dtype, uint32_t n_tx, uint32_t n_ch,
                     uint32_t n_s, double fs, double f_number, double c_mm_per_s, int window_type,
                     float window_parameter, int out_dtype, int precision, void* out, int nthreads) {
     if (!g || !rf || !out || n_tx == 0 || n_ch == 0 || n_s == 0) return -1;
-    if (g->rx_scanline_layout[0] > 1 && g->rx_scanline_layout[1] > 1) return -3;
+    const bool is3d = g->rx_scanline_layout[0] > 1 && g->rx_scanline_layout[1] > 1;
     for (uint32_t i = 0; i < g->num_rx_scanlines; i++) {
         const int k = select_tx(g->scanlines[i]);
         const supra_tx_params& t = g->scanlines[i].tx[k];
         if (g->scanlines[i].tx_weights[k] > 0.0 && t.tx_scanline_idx >= n_tx) return -2;
-        if (t.last_active_element[0] > t.first_active_element[0] &&
+        if (!is3d && t.last_active_element[0] > t.first_active_element[0] &&
             static_cast<uint32_t>(t.last_active_element[0] - t.first_active_element[0]) > n_ch) return -2;
     }
     DasCall call{g, RfView{rf, rf_dtype, n_tx, n_ch, n_s}, fs, f_number, c_mm_per_s, window_type, window_parameter,
@@ -854,6 +1187,50 @@ int oracle_das_adjoint(const supra_rx_geometry* g, const float* rf, uint32_t n_f
                                             norm_out ? static_cast<float*>(norm_out) + norm_frame * f : nullptr);
         }
     }, 2);
+    return 0;
+}
+
+// Per-pixel decision differences of the 2D DAS between evaluation A (speed
+// of sound c_a, precision prec_a) and B (c_b, prec_b) (das_decision_diff):
+// flags uint8 [depth][scanline].
+int oracle_das_decision_diff(const supra_rx_geometry* g, uint32_t n_s, double fs, double f_number, double c_a,
+                             int prec_a, double c_b, int prec_b, int window_type, float window_parameter,
+                             int interpolate_transmits, int32_t additional_offset, uint8_t* flags, int nthreads) {
+    if (!g || !flags || n_s == 0) return -1;
+    if (g->rx_scanline_layout[0] > 1 && g->rx_scanline_layout[1] > 1) return -3;
+    if ((prec_a != 32 && prec_a != 64) || (prec_b != 32 && prec_b != 64)) return -1;
+    DasCall call{g, RfView{nullptr, SUPRA_FLOAT32, 0, 0, n_s}, fs, f_number, c_a, window_type,
+                 window_parameter, interpolate_transmits, additional_offset, 0, SUPRA_FLOAT32};
+    das_decision_diff(call, DecisionEval{c_a, prec_a}, DecisionEval{c_b, prec_b}, flags, nthreads);
+    return 0;
+}
+
+// Envelope of an image [sample][scanline] (FP32: precision 32, FP64: 64),
+// before the output cast.
+int oracle_envelope(const void* in, int in_dtype, uint32_t n_rx, uint32_t n_s, uint32_t filter_length,
+                    int precision, void* out, int nthreads) {
+    if (!in || !out || n_rx == 0 || n_s == 0 || filter_length == 0) return -1;
+    if (precision == 64) envelope_run<double>(in, in_dtype, n_rx, n_s, filter_length, static_cast<double*>(out), nthreads);
+    else envelope_run<float>(in, in_dtype, n_rx, n_s, filter_length, static_cast<float*>(out), nthreads);
+    return 0;
+}
+
+// Hilbert FIR coefficients (precision 32 or 64).
+int oracle_hilbert_filter(uint32_t filter_length, int precision, void* out) {
+    if (!out || filter_length == 0) return -1;
+    if (precision == 64) hilbert_filter<double>(filter_length, static_cast<double*>(out));
+    else hilbert_filter<float>(filter_length, static_cast<float*>(out));
+    return 0;
+}
+
+// Log compression of `count` values, before the output cast.
+int oracle_logcompress(const void* in, int in_dtype, uint64_t count, double dynamic_range_db, double gain,
+                       double in_max, int precision, void* out, int nthreads) {
+    if (!in || !out) return -1;
+    if (precision == 64)
+        logcompress_run<double>(in, in_dtype, count, dynamic_range_db, gain, in_max, static_cast<double*>(out), nthreads);
+    else
+        logcompress_run<float>(in, in_dtype, count, dynamic_range_db, gain, in_max, static_cast<float*>(out), nthreads);
     return 0;
 }
 

@@ -47,12 +47,25 @@ def lib() -> C.CDLL:
         h.oracle_mv.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.c_double, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int, C.c_uint32,
                                 C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        h.oracle_envelope.restype = C.c_int
+        h.oracle_envelope.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p,
+                                      C.c_int]
+        h.oracle_hilbert_filter.restype = C.c_int
+        h.oracle_hilbert_filter.argtypes = [C.c_uint32, C.c_int, C.c_void_p]
+        h.oracle_logcompress.restype = C.c_int
+        h.oracle_logcompress.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                         C.c_int, C.c_void_p, C.c_int]
+        h.oracle_das_decision_diff.restype = C.c_int
+        h.oracle_das_decision_diff.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_int,
+                                               C.c_double, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int32,
+                                               C.c_void_p, C.c_int]
         _lib = h
     return _lib
 
 
 def default_threads() -> int:
-    return max(1, min(16, os.cpu_count() or 1))
+    """Every core this process may run on."""
+    return max(1, len(os.sched_getaffinity(0)))
 
 
 def window_lut(window_type: int, window_parameter: float, precision: int = 32) -> np.ndarray:
@@ -64,9 +77,12 @@ def window_lut(window_type: int, window_parameter: float, precision: int = 32) -
 def das(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_per_s: float,
         window_type: int, window_parameter: float = 0.0, interpolate_transmits: bool = False,
         additional_offset: int = 0, test_signal: bool = False, out_int16: bool = False,
-        precision: int = 32, pixels: np.ndarray | None = None, threads: int | None = None) -> np.ndarray:
+        precision: int = 32, pixels: np.ndarray | None = None, threads: int | None = None,
+        sample_beamformer: int = 0) -> np.ndarray:
     """Oracle DAS image [num_depths, num_rx] (or the values at `pixels`,
-    flat indices depth * num_rx + scanline).  `geom` is a RxGeometry."""
+    flat indices depth * num_rx + scanline).  `geom` is a RxGeometry.
+    sample_beamformer: 0 DelayAndSum, 1 DelayAndStdDev, 2
+    CoherenceFactorDelayAndSum (test_signal=True selects 3, TestSignal)."""
     rf = np.ascontiguousarray(rf)
     assert rf.ndim == 3 and rf.dtype in (np.int16, np.float32)
     gc = geom.to_c()
@@ -81,7 +97,8 @@ def das(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_p
     code = lib().oracle_das(C.addressof(gc), rf.ctypes.data, 1 if rf.dtype == np.int16 else 2,
                             rf.shape[0], rf.shape[1], rf.shape[2], fs_hz, f_number, speed_of_sound_mm_per_s,
                             window_type, window_parameter, int(interpolate_transmits), additional_offset,
-                            int(test_signal), 1 if out_int16 else 2, precision, pix_ptr, npix,
+                            3 if test_signal else int(sample_beamformer), 1 if out_int16 else 2, precision,
+                            pix_ptr, npix,
                             out.ctypes.data, threads or default_threads())
     if code != 0:
         raise ValueError(f"oracle_das rejected its arguments ({code})")
@@ -118,10 +135,30 @@ def das_adjoint(geom, rf: np.ndarray, fs_hz: float, f_number: float, c_mm_per_s,
     return grad_rf, grad_c, norm
 
 
+def das_decision_diff(geom, num_samples: int, fs_hz: float, f_number: float, speed_of_sound_mm_per_s: float,
+                      window_type: int, window_parameter: float = 0.0, interpolate_transmits: bool = False,
+                      additional_offset: int = 0, threads: int | None = None, precision_a: int = 32,
+                      precision_b: int = 64, speed_of_sound_b_mm_per_s: float | None = None) -> np.ndarray:
+    """uint8 [num_depths, num_rx]: where two evaluations of the 2D DAS take
+    different decisions for some element (1 membership, 2 window value, 4
+    floor of the delay, 8 range test).  By default FP32 against FP64 at one
+    speed of sound; speed_of_sound_b_mm_per_s / precision_b give evaluation B."""
+    gc = geom.to_c()
+    out = np.zeros((geom.num_depths, geom.num_rx), dtype=np.uint8)
+    c_b = speed_of_sound_mm_per_s if speed_of_sound_b_mm_per_s is None else speed_of_sound_b_mm_per_s
+    code = lib().oracle_das_decision_diff(C.addressof(gc), num_samples, fs_hz, f_number, speed_of_sound_mm_per_s,
+                                          precision_a, c_b, precision_b, window_type, window_parameter,
+                                          int(interpolate_transmits), additional_offset, out.ctypes.data,
+                                          threads or default_threads())
+    if code != 0:
+        raise ValueError(f"oracle_das_decision_diff rejected its arguments ({code})")
+    return out
+
+
 def rawdelay(geom, rf: np.ndarray, fs_hz: float, f_number: float, speed_of_sound_mm_per_s: float,
              window_type: int, window_parameter: float = 0.0, out_int16: bool = False, precision: int = 32,
              threads: int | None = None) -> np.ndarray:
-    """Oracle pre-delay [num_rx, num_channels, num_depths] (2D geometries)."""
+    """Oracle pre-delay [num_rx, num_channels, num_depths] (2D and 3D geometries)."""
     rf = np.ascontiguousarray(rf)
     gc = geom.to_c()
     out = np.zeros((geom.num_rx, rf.shape[1], geom.num_depths), dtype=np.float32 if precision == 32 else np.float64)
@@ -158,3 +195,51 @@ def mv(x: np.ndarray, sub_array_size: int = 0, temporal_smoothing: int = 3, scal
         n = int(dm[2])
         return out, dA[:n * n].reshape(n, n), dw[:n], tuple(int(v) for v in dm)
     return out
+
+
+_DTYPE = {np.dtype(np.int16): 1, np.dtype(np.float32): 2, np.dtype(np.uint8): 3}
+
+
+def hilbert_filter(filter_length: int, precision: int = 32) -> np.ndarray:
+    """Hilbert transformer FIR taps with the Hamming window (FirFilterFactory)."""
+    out = np.zeros(filter_length, dtype=np.float32 if precision == 32 else np.float64)
+    assert lib().oracle_hilbert_filter(filter_length, precision, out.ctypes.data) == 0
+    return out
+
+
+def envelope(image: np.ndarray, filter_length: int = 65, precision: int = 32, threads: int | None = None):
+    """Envelope sqrt(x^2 + H{x}^2) of an image [sample][scanline] (before the
+    output cast), HilbertFirEnvelope's FIR along the samples."""
+    image = np.ascontiguousarray(image)
+    n_s, n_rx = image.shape
+    out = np.zeros(image.shape, dtype=np.float32 if precision == 32 else np.float64)
+    code = lib().oracle_envelope(image.ctypes.data, _DTYPE[image.dtype], n_rx, n_s, filter_length, precision,
+                                 out.ctypes.data, threads or default_threads())
+    if code != 0:
+        raise ValueError(f"oracle_envelope rejected its arguments ({code})")
+    return out
+
+
+def logcompress(values: np.ndarray, dynamic_range_db: float = 80.0, gain: float = 1.0, in_max: float = 1024.0,
+                precision: int = 32, threads: int | None = None):
+    """LogCompressor's log10(|x| inScale + 1) * gain * 255 / log10(DR + 1)
+    (before the output cast), same shape as values."""
+    values = np.ascontiguousarray(values)
+    out = np.zeros(values.shape, dtype=np.float32 if precision == 32 else np.float64)
+    code = lib().oracle_logcompress(values.ctypes.data, _DTYPE[values.dtype], values.size, dynamic_range_db, gain,
+                                    in_max, precision, out.ctypes.data, threads or default_threads())
+    if code != 0:
+        raise ValueError(f"oracle_logcompress rejected its arguments ({code})")
+    return out
+
+
+def clamp_cast(values: np.ndarray, dtype) -> np.ndarray:
+    """clampCast (utilities/cudaUtility.h:111-147): clamp to the type's
+    limits (int16: +-32767, uint8: 0..255, float: +-FLT_MAX; NaN -> the lower
+    limit) and truncate."""
+    v = np.asarray(values)
+    lo, hi = {np.dtype(np.int16): (-32767.0, 32767.0), np.dtype(np.uint8): (0.0, 255.0),
+              np.dtype(np.float32): (-3.4028234663852886e38, 3.4028234663852886e38)}[np.dtype(dtype)]
+    c = np.fmin(np.fmax(v, v.dtype.type(lo)), v.dtype.type(hi))
+    c = np.where(np.isnan(v), lo, c)
+    return c.astype(dtype)

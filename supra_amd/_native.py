@@ -15,13 +15,23 @@ LIB_PATH = Path(__file__).resolve().parent / "libsupra_bf.so"
 
 # status codes / enums (supra_bf.h)
 SUPRA_OK = 0
+SUPRA_ERR_INVALID_ARG, SUPRA_ERR_UNSUPPORTED, SUPRA_ERR_HIP, SUPRA_ERR_NO_DEVICE, SUPRA_ERR_TOO_LARGE = -1, -2, -3, -4, -5
 SUPRA_INT16 = 1
 SUPRA_FLOAT32 = 2
+SUPRA_UINT8 = 3
 WINDOW_RECTANGULAR, WINDOW_HANN, WINDOW_HAMMING, WINDOW_GAUSS = 0, 1, 2, 3
-BF_DELAY_AND_SUM, BF_TEST_SIGNAL = 0, 3
+BF_DELAY_AND_SUM, BF_DELAY_AND_STD_DEV, BF_COHERENCE_FACTOR, BF_TEST_SIGNAL = 0, 1, 2, 3
 SCAN_LINEAR, SCAN_PHASED, SCAN_BIPHASED, SCAN_PLANEWAVE = 0, 1, 2, 3
 TRANSDUCER_LINEAR, TRANSDUCER_PLANAR = 0, 1
 WINDOW_NUM_ENTRIES = 64
+
+
+class MockMetaC(C.Structure):
+    """supra_mock_meta (supra_bf.h)."""
+    _fields_ = [("num_elements", C.c_uint32), ("element_layout", C.c_uint32 * 2), ("num_channels", C.c_uint32),
+                ("num_samples", C.c_uint32), ("num_tx", C.c_uint32), ("rx_scanline_layout", C.c_uint32 * 2),
+                ("num_depths", C.c_uint32), ("depth", C.c_double), ("sampling_frequency_hz", C.c_double),
+                ("speed_of_sound_mm_per_s", C.c_double)]
 
 
 class SupraError(RuntimeError):
@@ -144,6 +154,19 @@ _SIGNATURES = {
                                    C.c_double, C.c_double, C.c_double, C.c_int32, C.c_float, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_int32, C.c_double, C.c_void_p, C.c_int32,
                                    C.c_void_p]),
+    "supra_mock_read_meta": (C.c_int, [C.c_char_p, C.c_void_p]),
+    "supra_mock_read_geometry": (C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "supra_mock_write_json": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(RxGeometryC)]),
+    "supra_mock_write_ascii": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(RxGeometryC)]),
+    "supra_mock_count_frames": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "supra_mock_read_frames": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
+    "supra_envelope_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                     C.c_int32, C.c_void_p]),
+    "supra_logcompress_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                        C.c_void_p, C.c_int32, C.c_void_p]),
+    "supra_envelope_logcompress_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int32,
+                                                 C.c_void_p]),
     "supra_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.POINTER(C.c_void_p)]),
     "supra_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "supra_batch_latencies": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

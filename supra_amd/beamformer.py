@@ -28,8 +28,8 @@ except ImportError:  # pragma: no cover - torch is part of the image
 
 WINDOW_TYPES = {"Rectangular": N.WINDOW_RECTANGULAR, "Hann": N.WINDOW_HANN,
                 "Hamming": N.WINDOW_HAMMING, "Gauss": N.WINDOW_GAUSS}
-BEAMFORMER_TYPES = {"DelayAndSum": N.BF_DELAY_AND_SUM, "DelayAndStdDev": 1,
-                    "CoherenceFactorDelayAndSum": 2, "TestSignal": N.BF_TEST_SIGNAL}
+BEAMFORMER_TYPES = {"DelayAndSum": N.BF_DELAY_AND_SUM, "DelayAndStdDev": N.BF_DELAY_AND_STD_DEV,
+                    "CoherenceFactorDelayAndSum": N.BF_COHERENCE_FACTOR, "TestSignal": N.BF_TEST_SIGNAL}
 OUTPUT_TYPES = {"float": N.SUPRA_FLOAT32, "int16": N.SUPRA_INT16}
 
 
@@ -203,8 +203,6 @@ class BeamformingNode:
             self._engine = RxBeamformer(raw.geometry)
             self._engine_geometry = raw.geometry
         bt = BEAMFORMER_TYPES[self.config["beamformerType"]]
-        if bt not in (N.BF_DELAY_AND_SUM, N.BF_TEST_SIGNAL):
-            raise NotImplementedError(f"beamformerType {self.config['beamformerType']} is not provided")
         out_dtype = torch.float32 if self.config["outputType"] == "float" else torch.int16
         img = self._engine.perform_rx_beamforming(
             bt, raw, self.config["fNumber"], self.config["speedOfSound"] * 1000.0,

@@ -23,8 +23,9 @@ LIB = PKG / "libsupra_bf.so"
 
 ARCH = os.environ.get("SUPRA_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["geometry.cpp", "bf_host.cpp", "status.cpp"]
-HIP_SOURCES = ["bf_context.hip", "das_kernels.hip", "das_adjoint.hip", "mv_kernels.hip", "synth.hip", "batch.hip"]
+HOST_SOURCES = ["geometry.cpp", "bf_host.cpp", "status.cpp", "mock_io.cpp"]
+HIP_SOURCES = ["bf_context.hip", "das_kernels.hip", "das_adjoint.hip", "mv_kernels.hip", "synth.hip", "batch.hip",
+               "envelope.hip", "das_variants.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_FLAGS = [

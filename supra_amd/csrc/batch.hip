@@ -93,6 +93,7 @@ struct supra_batch {
     void* tables = nullptr;            // per-frame dt-space tables
     uint32_t* sync = nullptr;          // heads[kMaxQueues] + done[n_frames] (uint32) + stamps (uint64)
     uint64_t* stamp = nullptr;
+    bool launched = false;             // supra_batch_launch has run (the stamps are meaningful)
     uint32_t* heads = nullptr;
     uint32_t* done = nullptr;
     std::string config;
@@ -172,7 +173,7 @@ int check_item(const supra_batch_item& it, uint32_t index) {
         std::snprintf(what, sizeof(what), "batch item %u", index);
         return check_rawdelay_call(ctx, what, it.rf_dtype, it.num_tx, it.num_channels, it.num_samples,
                                    it.sampling_frequency_hz, it.f_number, it.speed_of_sound_mm_per_s,
-                                   it.window_type);
+                                   it.window_type, false);
     }
     return fail(SUPRA_ERR_INVALID_ARG, "batch item %u: unknown mode %d", index, it.mode);
 }
@@ -504,6 +505,8 @@ extern "C" int supra_batch_create(const supra_batch_item* items, uint32_t num_it
     const size_t sync_bytes =
         align256((kMaxQueues + num_items) * sizeof(uint32_t)) + (num_items + 1) * sizeof(uint64_t);
     if ((e = hipMalloc(&b->sync, sync_bytes)) != hipSuccess) return hip_fail(e, "batch: counters");
+    // (not captured: the graph clears the counters with its reset kernel)
+    if ((e = hipMemset(b->sync, 0, sync_bytes)) != hipSuccess) return hip_fail(e, "batch: counters");
     b->heads = b->sync;
     b->done = b->sync + kMaxQueues;
     b->stamp = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(b->sync) +
@@ -552,12 +555,14 @@ extern "C" int supra_batch_launch(supra_batch* b, void* stream) {
     const hipError_t e = hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "batch: graph launch");
     b->last_stream = static_cast<hipStream_t>(stream);
+    b->launched = true;
     return SUPRA_OK;
 }
 
 extern "C" int supra_batch_latencies(supra_batch* b, double* frame_us, double* total_us) {
     clear_error();
     if (!b) return fail(SUPRA_ERR_INVALID_ARG, "batch: null argument");
+    if (!b->launched) return fail(SUPRA_ERR_INVALID_ARG, "batch: latencies before the first supra_batch_launch");
     hipError_t e = hipStreamSynchronize(b->last_stream);
     if (e != hipSuccess) return hip_fail(e, "batch: synchronize");
     std::vector<uint64_t> st(b->n_frames + 1);

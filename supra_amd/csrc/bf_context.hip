@@ -97,6 +97,10 @@ int ensure_dt_tables(supra_bf_ctx* ctx, double dt, double c, hipStream_t stream)
     if (ctx->cached_dt == dt && ctx->cached_c == c) return SUPRA_OK;
     build_dt_tables(ctx, dt, c);
     hipError_t e;
+    // Kernels of earlier calls (possibly on other streams) may still read the
+    // tables: the upload waits for the last of them.
+    if (ctx->tables_used_recorded && (e = hipStreamWaitEvent(stream, ctx->tables_used, 0)) != hipSuccess)
+        return hip_fail(e, "geometry upload: wait for earlier kernels");
     if ((e = hipMemcpyAsync(ctx->d_scans, ctx->h_scans.data(), ctx->n_rx * sizeof(ScanlineDT), hipMemcpyHostToDevice,
                             stream)) != hipSuccess)
         return hip_fail(e, "upload scanlines");
@@ -113,6 +117,14 @@ int ensure_dt_tables(supra_bf_ctx* ctx, double dt, double c, hipStream_t stream)
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "geometry upload");
     ctx->cached_dt = dt;
     ctx->cached_c = c;
+    return SUPRA_OK;
+}
+
+// Marks the end of a launch that reads the context's dt-space tables.
+int tables_used_on(supra_bf_ctx* ctx, hipStream_t stream) {
+    hipError_t e = hipEventRecord(ctx->tables_used, stream);
+    if (e != hipSuccess) return hip_fail(e, "record table use");
+    ctx->tables_used_recorded = true;
     return SUPRA_OK;
 }
 
@@ -194,6 +206,10 @@ extern "C" int supra_bf_create(const supra_rx_geometry* g, supra_bf_ctx** out_ct
         delete ctx;
         return fail(SUPRA_ERR_NO_DEVICE, "create: no HIP device: %s", hipGetErrorString(e));
     }
+    if ((e = hipEventCreateWithFlags(&ctx->tables_used, hipEventDisableTiming)) != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "create: event");
+    }
     if ((e = hipMalloc(&ctx->d_scans, ctx->n_rx * sizeof(ScanlineDT))) != hipSuccess ||
         (e = hipMalloc(&ctx->d_depths, ctx->n_depths * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&ctx->d_ex, ctx->n_el * sizeof(float))) != hipSuccess ||
@@ -227,9 +243,10 @@ extern "C" int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer
     if ((rf_dtype != SUPRA_INT16 && rf_dtype != SUPRA_FLOAT32) ||
         (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
         return fail(SUPRA_ERR_UNSUPPORTED, "das: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
-    if (sample_beamformer != SUPRA_BF_DELAY_AND_SUM && sample_beamformer != SUPRA_BF_TEST_SIGNAL)
-        return fail(SUPRA_ERR_UNSUPPORTED, "das: sample beamformer %d is not provided (DelayAndSum=0, TestSignal=3)",
-                    sample_beamformer);
+    if (sample_beamformer < SUPRA_BF_DELAY_AND_SUM || sample_beamformer > SUPRA_BF_TEST_SIGNAL)
+        return fail(SUPRA_ERR_UNSUPPORTED,
+                    "das: unknown sample beamformer %d (DelayAndSum=0, DelayAndStdDev=1, "
+                    "CoherenceFactorDelayAndSum=2, TestSignal=3)", sample_beamformer);
     if (window_type < SUPRA_WINDOW_RECTANGULAR || window_type > SUPRA_WINDOW_GAUSS)
         return fail(SUPRA_ERR_INVALID_ARG, "das: unknown window type %d", window_type);
     if (num_tx == 0 || num_channels == 0 || num_samples == 0)
@@ -297,10 +314,14 @@ extern "C" int supra_das_run_frames(supra_bf_ctx* ctx, int32_t sample_beamformer
     a.far_ok = ctx->far_ok ? 1 : 0;
     build_window_lut(window_type, window_parameter, a.lut);
 
-    hipError_t e = launch_das(a, rf_dtype, out_dtype, ctx->is3d, sample_beamformer == SUPRA_BF_TEST_SIGNAL,
-                              interpolate_transmits != 0, ctx->d_e2c != nullptr, hs);
+    hipError_t e;
+    if (sample_beamformer == SUPRA_BF_DELAY_AND_STD_DEV || sample_beamformer == SUPRA_BF_COHERENCE_FACTOR)
+        e = launch_das_variant(a, rf_dtype, out_dtype, ctx->is3d, sample_beamformer, interpolate_transmits != 0, hs);
+    else
+        e = launch_das(a, rf_dtype, out_dtype, ctx->is3d, sample_beamformer == SUPRA_BF_TEST_SIGNAL,
+                       interpolate_transmits != 0, ctx->d_e2c != nullptr, hs);
     if (e != hipSuccess) return hip_fail(e, "das: launch");
-    return SUPRA_OK;
+    return tables_used_on(ctx, hs);
 }
 
 extern "C" int supra_das_run(supra_bf_ctx* ctx, int32_t sample_beamformer, const void* rf, int32_t rf_dtype,
@@ -318,10 +339,10 @@ namespace supra_amd {
 // Checks of the pre-delay entry points (RawDelay::performDelay preconditions).
 int check_rawdelay_call(const supra_bf_ctx* ctx, const char* what, int32_t rf_dtype, uint32_t num_tx,
                         uint32_t num_channels, uint32_t num_samples, double fs, double f_number, double c,
-                        int32_t window_type) {
+                        int32_t window_type, bool allow3d) {
     if (rf_dtype != SUPRA_INT16 && rf_dtype != SUPRA_FLOAT32)
         return fail(SUPRA_ERR_UNSUPPORTED, "%s: element types must be SUPRA_INT16 or SUPRA_FLOAT32", what);
-    if (ctx->is3d) return fail(SUPRA_ERR_UNSUPPORTED, "%s: 3D geometries are not provided (2D pre-delay only)", what);
+    if (ctx->is3d && !allow3d) return fail(SUPRA_ERR_UNSUPPORTED, "%s: 2D geometries only", what);
     if (window_type < SUPRA_WINDOW_RECTANGULAR || window_type > SUPRA_WINDOW_GAUSS)
         return fail(SUPRA_ERR_INVALID_ARG, "%s: unknown window type %d", what, window_type);
     if (num_tx == 0 || num_channels == 0 || num_samples == 0) return fail(SUPRA_ERR_INVALID_ARG, "%s: empty RF", what);
@@ -342,10 +363,20 @@ int check_rawdelay_call(const supra_bf_ctx* ctx, const char* what, int32_t rf_dt
         if (t.tx_scanline_idx >= num_tx)
             return fail(SUPRA_ERR_INVALID_ARG, "%s: scanline %u uses transmit %u of %u", what, i,
                         static_cast<unsigned>(t.tx_scanline_idx), num_tx);
-        if (t.last_active_element[0] > t.first_active_element[0] &&
-            static_cast<uint32_t>(t.last_active_element[0] - t.first_active_element[0]) > num_channels)
+        if (ctx->is3d) {
+            // slot (ex - first.x) + (ey - first.y) * layout.x (RawDelay.cu:207-209)
+            if (t.last_active_element[0] > t.first_active_element[0] &&
+                t.last_active_element[1] > t.first_active_element[1] &&
+                static_cast<uint64_t>(t.last_active_element[0] - t.first_active_element[0] - 1) +
+                        static_cast<uint64_t>(t.last_active_element[1] - t.first_active_element[1] - 1) *
+                            ctx->el_layout[0] >= num_channels)
+                return fail(SUPRA_ERR_INVALID_ARG, "%s: scanline %u aperture exceeds the %u output channels", what, i,
+                            num_channels);
+        } else if (t.last_active_element[0] > t.first_active_element[0] &&
+                   static_cast<uint32_t>(t.last_active_element[0] - t.first_active_element[0]) > num_channels) {
             return fail(SUPRA_ERR_INVALID_ARG, "%s: scanline %u aperture exceeds the %u output channels", what, i,
                         num_channels);
+        }
     }
     return SUPRA_OK;
 }
@@ -393,7 +424,7 @@ extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_
     if (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32)
         return fail(SUPRA_ERR_UNSUPPORTED, "rawdelay: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
     int st = check_rawdelay_call(ctx, "rawdelay", rf_dtype, num_tx, num_channels, num_samples, sampling_frequency_hz,
-                                 f_number, speed_of_sound_mm_per_s, window_type);
+                                 f_number, speed_of_sound_mm_per_s, window_type, true);
     if (st != SUPRA_OK) return st;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if ((st = check_device(ctx)) != SUPRA_OK) return st;
@@ -403,8 +434,9 @@ extern "C" int supra_rawdelay_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_
     DasArgs a = rawdelay_args(ctx, rf, num_tx, num_channels, num_samples, dt, f_number, speed_of_sound_mm_per_s,
                               window_type, window_parameter);
     a.out = out;
-    hipError_t e = launch_rawdelay(a, rf_dtype, out_dtype, hs);
+    hipError_t e = launch_rawdelay(a, rf_dtype, out_dtype, ctx->is3d, hs);
     if (e != hipSuccess) return hip_fail(e, "rawdelay: launch");
+    if ((st = tables_used_on(ctx, hs)) != SUPRA_OK) return st;
     if (out_sampling_frequency_hz) {
         // RawDelay.cu:48, :642: depth of the last sample / c / N_d.
         const double depth = ctx->depths_mm.back();
@@ -424,7 +456,7 @@ extern "C" int supra_mv_das_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dt
     if (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32)
         return fail(SUPRA_ERR_UNSUPPORTED, "mv: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
     int st = check_rawdelay_call(ctx, "mv", rf_dtype, num_tx, num_channels, num_samples, sampling_frequency_hz,
-                                 f_number, speed_of_sound_mm_per_s, window_type);
+                                 f_number, speed_of_sound_mm_per_s, window_type, false);
     if (st != SUPRA_OK) return st;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if ((st = check_device(ctx)) != SUPRA_OK) return st;
@@ -433,8 +465,10 @@ extern "C" int supra_mv_das_run(supra_bf_ctx* ctx, const void* rf, int32_t rf_dt
     if ((st = ensure_dt_tables(ctx, dt, speed_of_sound_mm_per_s, hs)) != SUPRA_OK) return st;
     const DasArgs g = rawdelay_args(ctx, rf, num_tx, num_channels, num_samples, dt, f_number,
                                     speed_of_sound_mm_per_s, window_type, window_parameter);
-    return launch_mv_fused(g, rf_dtype, sub_array_size, temporal_smoothing, sub_array_scaling_power, compute_means,
-                           loading_factor, out, out_dtype, hs);
+    st = launch_mv_fused(g, rf_dtype, sub_array_size, temporal_smoothing, sub_array_scaling_power, compute_means,
+                         loading_factor, out, out_dtype, hs);
+    if (st != SUPRA_OK) return st;
+    return tables_used_on(ctx, hs);
 }
 
 // ---- differentiable 2D DAS -----------------------------------------------------

@@ -31,6 +31,10 @@ struct supra_bf_ctx {
     std::vector<ScanlineDT> h_scans;
     std::vector<float> h_depths, h_ex, h_ey;
     double cached_dt = 0, cached_c = 0;
+    // Recorded after every launch that reads d_scans/d_depths/d_ex/d_ey; a
+    // table upload for another (dt, c) waits for it.
+    hipEvent_t tables_used = nullptr;
+    bool tables_used_recorded = false;
     bool far_ok = false;   // dt-space depths are >= 0 and non-decreasing
     bool far_ok_mm = false;   // the same for the millimetre depths (any c)
     std::mutex mu;
@@ -58,6 +62,7 @@ struct supra_bf_ctx {
     uint32_t max_aperture = 0;   // largest last_x - first_x over scanlines and transmits
 
     ~supra_bf_ctx() {
+        if (tables_used) (void)hipEventDestroy(tables_used);
         (void)hipFree(d_scans);
         (void)hipFree(d_depths);
         (void)hipFree(d_ex);
@@ -83,9 +88,10 @@ int hip_fail(hipError_t e, const char* what);
 void build_dt_tables(supra_bf_ctx* ctx, double dt, double c);
 int ensure_dt_tables(supra_bf_ctx* ctx, double dt, double c, hipStream_t stream);
 int check_device(const supra_bf_ctx* ctx);
+int tables_used_on(supra_bf_ctx* ctx, hipStream_t stream);
 int check_rawdelay_call(const supra_bf_ctx* ctx, const char* what, int32_t rf_dtype, uint32_t num_tx,
                         uint32_t num_channels, uint32_t num_samples, double fs, double f_number, double c,
-                        int32_t window_type);
+                        int32_t window_type, bool allow3d);
 DasArgs rawdelay_args(const supra_bf_ctx* ctx, const void* rf, uint32_t num_tx, uint32_t num_channels,
                       uint32_t num_samples, double dt, double f_number, double c, int32_t window_type,
                       float window_parameter);

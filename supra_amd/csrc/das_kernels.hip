@@ -55,6 +55,11 @@ constexpr int kChunk2D = 128;     // aperture elements staged per LDS pass (2D, 
 // tiles, a tenth of the lanes idle in the last one).
 constexpr int kWaveDepths = 64;
 constexpr int kWavesPerGroup = kThreads / 64;
+// Frames per wave of the multi-frame 2D kernel (frames sharing a geometry).
+#ifndef SUPRA_FRAMES_PER_WAVE
+#define SUPRA_FRAMES_PER_WAVE 8
+#endif
+constexpr int kFramesPerWave = SUPRA_FRAMES_PER_WAVE;
 // Aperture elements staged per LDS pass (3D).  Small on purpose: the 3D
 // kernel is bound by gather latency, and a small LDS footprint per
 // workgroup (about 12 KB with the packed window table) lets 10 workgroups
@@ -289,6 +294,179 @@ __global__ __launch_bounds__(kThreads) void das2d_kernel(const DasArgs a) {
     __shared__ Lds2D lds[kWavesPerGroup];
     const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
     das2d_tile<RFT, OutT, TESTSIG, INTERP_TX, MAP>(a, a.lut, decode_wave_2d(a), lds[wave]);
+}
+
+// ---- 2D, several frames per wave ---------------------------------------------
+//
+// Frames that share one geometry and one set of parameters take the same
+// decisions for a pixel: aperture membership, window weight, W and n, the
+// delay, its floor and fraction and the range test depend only on the
+// geometry.  One wave therefore takes one 64-depth unit of one scanline for
+// FB frames at once: it evaluates an element's delay (the correctly rounded
+// square root) and decisions once, then gathers and interpolates the
+// element's two samples in each of the FB frames.  Every frame's arithmetic
+// is the single-frame kernel's, in the same order, so the images are the
+// same bits.
+
+// Elements [0, cnt) of the current LDS chunk for one 2D pixel in FB frames.
+template <typename RFT, int FB, bool FAR, bool VERT, bool MAP>
+__device__ inline void elements2d_frames(const __amdgpu_buffer_rsrc_t (&rs)[FB], uint32_t n_s, float t0,
+                                         float offset, const Pixel2D& p, const Lds2D& L, int cnt,
+                                         float (&sample)[FB], float& wsum, int& nadd) {
+    const float last = static_cast<float>(n_s - 1u);
+#pragma unroll 2
+    for (int j = 0; j < cnt; j++) {
+        const uint32_t off = L.off[j];
+        if (MAP && off == kNotConnected) continue;
+        float w;
+        if (FAR) {
+            w = L.w[j];
+        } else {
+            const float dx = L.dx[j];
+            if (!(fabsf(dx) <= p.aDT)) continue;
+            w = L.lut[window_index(dx * p.inv)];
+        }
+        wsum += w;
+        nadd++;
+        float q;
+        if (VERT) {
+            q = L.dx2[j] + p.py2;
+        } else {
+            const float e = L.xe[j] - p.px;
+            q = e * e + p.py2;
+        }
+        float tau = t0 + (sqrt_rn(q) + p.d);
+        tau = tau + offset;
+        const uint32_t ui = static_cast<uint32_t>(static_cast<int32_t>(floorf(tau)));
+        const float f = __builtin_amdgcn_fractf(tau);
+        const float omf = 1.0f - f;
+        const uint32_t idx = off + ui;
+        if constexpr (std::is_same_v<RFT, int16_t>) {
+            // 0 <= tau <= N-1 (see gather()); outside it the element adds
+            // w_eff * val = +-0 to the sum, which leaves it unchanged (int16
+            // samples are finite).
+            const bool in = (tau >= 0.0f) & (tau <= last);
+            const float w_eff = in ? w : 0.0f;
+            float v0[FB], v1[FB];
+#pragma unroll
+            for (int fb = 0; fb < FB; fb++) {     // every frame's gathers in flight at once
+                v0[fb] = load_rf<RFT>(rs[fb], idx);
+                v1[fb] = load_rf<RFT>(rs[fb], idx + 1u);
+            }
+#pragma unroll
+            for (int fb = 0; fb < FB; fb++) sample[fb] += w_eff * (omf * v0[fb] + f * v1[fb]);
+        } else {
+            const bool inr = ui < n_s - 1u;
+            const bool edge = (ui == n_s - 1u) & (f == 0.0f);
+            float v0[FB], v1[FB];
+#pragma unroll
+            for (int fb = 0; fb < FB; fb++) {
+                v0[fb] = load_rf<RFT>(rs[fb], idx);
+                v1[fb] = load_rf<RFT>(rs[fb], idx + 1u);
+            }
+#pragma unroll
+            for (int fb = 0; fb < FB; fb++) {
+                const float val = edge ? v0[fb] : (omf * v0[fb] + f * v1[fb]);
+                const float contrib = w * val;
+                sample[fb] = (inr | edge) ? sample[fb] + contrib : sample[fb];
+            }
+        }
+    }
+}
+
+// One wave: 64 consecutive depths of one scanline in frames f0 .. f0+FB-1
+// (the last group may hold fewer frames; its missing frames re-read the last
+// frame and are not written).  DelayAndSum without transmit interpolation,
+// one geometry for all frames.
+template <typename RFT, typename OutT, int FB, bool MAP>
+__global__ __launch_bounds__(kThreads) void das2d_frames_kernel(const DasArgs a) {
+    __shared__ Lds2D lds[kWavesPerGroup];
+    const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
+    const int lane = static_cast<int>(threadIdx.x % 64);
+    const uint32_t chunks = (a.n_depths + kWaveDepths - 1) / kWaveDepths;
+    const uint32_t groups = (a.n_frames + FB - 1) / FB;
+    const uint64_t unit = static_cast<uint64_t>(blockIdx.x) * kWavesPerGroup + wave;
+    const uint64_t rest = unit / chunks;
+    const uint32_t s = static_cast<uint32_t>(rest % a.n_rx);
+    const uint64_t grp = rest / a.n_rx;
+    if (grp >= groups) return;                       // (wave-uniform) past the end of the unit list
+    const uint32_t r0 = static_cast<uint32_t>(unit % chunks) * kWaveDepths;
+    const uint32_t r = r0 + static_cast<uint32_t>(lane);
+    const bool valid = r < a.n_depths;
+    const uint32_t f0 = static_cast<uint32_t>(grp) * FB;
+    const uint32_t nfr = min(static_cast<uint32_t>(FB), a.n_frames - f0);
+    Lds2D& L = lds[wave];
+    for (int i = lane; i < kWindowEntries; i += 64) L.lut[i] = a.lut[i];
+
+    const ScanView sc = scan_view(a.scans, s);
+    Pixel2D p;
+    p.d = valid ? a.depths[r] : 0.0f;
+    p.aDT = (((p.d * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    p.inv = 1.0f / fminf(p.aDT, sc.max_ed_x());
+    p.px = sc.x() + sc.dir_x() * p.d;
+    const float py = sc.dir_y() * p.d;
+    p.py2 = py * py;
+    const bool vertical = (sc.dir_x() == 0.0f);
+    const float d0 = load_uniform_float(a.depths, r0);
+    const float aDT0 = (((d0 * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    const float inv_far = 1.0f / sc.max_ed_x();
+
+    float sample[FB];
+#pragma unroll
+    for (int fb = 0; fb < FB; fb++) sample[fb] = 0.0f;
+    float wsum = 0.0f;
+    int nadd = 0;
+    const int k = sc.sel();
+    if (k >= 0) {                                    // uniform
+        const TxDT tx = sc.tx(k);
+        const uint64_t tx_off = static_cast<uint64_t>(tx.tx_idx) * a.n_ch * a.n_s;
+        __amdgpu_buffer_rsrc_t rs[FB];
+#pragma unroll
+        for (int fb = 0; fb < FB; fb++) {
+            const uint32_t frame = f0 + min(static_cast<uint32_t>(fb), nfr - 1u);
+            const RFT* base = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(frame) * a.rf_frame_stride + tx_off;
+            rs[fb] = rf_resource(base, a.n_ch * a.n_s * sizeof(RFT));
+        }
+        const bool far = a.far_ok && aDT0 >= tx.far_thr;
+        const int first = tx.first_x, last = tx.last_x;
+        for (int base = first; base < last; base += kChunk2D) {
+            const int cnt = min(kChunk2D, last - base);
+            wave_sync_2d();
+            for (int j = lane; j < cnt; j += 64) {
+                const int e = base + j;
+                uint32_t ch;
+                if (MAP) {
+                    const int32_t m = a.e2c[e];
+                    ch = (m < 0) ? kNotConnected : static_cast<uint32_t>(m);
+                } else {
+                    ch = static_cast<uint32_t>(e) % a.n_ch;
+                }
+                const float xe = a.ex[e];
+                const float dx = xe - sc.x();
+                L.xe[j] = xe;
+                L.dx[j] = dx;
+                L.dx2[j] = dx * dx;
+                L.w[j] = L.lut[window_index(dx * inv_far)];
+                L.off[j] = (ch == kNotConnected) ? kNotConnected : ch * a.n_s;
+            }
+            wave_sync_2d();
+            if (far) {
+                if (vertical) elements2d_frames<RFT, FB, true, true, MAP>(rs, a.n_s, tx.t0, a.offset, p, L, cnt, sample, wsum, nadd);
+                else elements2d_frames<RFT, FB, true, false, MAP>(rs, a.n_s, tx.t0, a.offset, p, L, cnt, sample, wsum, nadd);
+            } else {
+                if (vertical) elements2d_frames<RFT, FB, false, true, MAP>(rs, a.n_s, tx.t0, a.offset, p, L, cnt, sample, wsum, nadd);
+                else elements2d_frames<RFT, FB, false, false, MAP>(rs, a.n_s, tx.t0, a.offset, p, L, cnt, sample, wsum, nadd);
+            }
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int fb = 0; fb < FB; fb++) {
+        if (static_cast<uint32_t>(fb) >= nfr) break;   // uniform
+        const float v = (nadd > 0) ? (sample[fb] / wsum) * static_cast<float>(nadd) : 0.0f;
+        OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(f0 + fb) * a.out_frame_stride;
+        out[static_cast<uint64_t>(r) * a.n_rx + s] = clamp_cast<OutT>(v);
+    }
 }
 
 // ---- 3D ---------------------------------------------------------------------
@@ -544,6 +722,111 @@ __global__ __launch_bounds__(kThreads) void rawdelay2d_kernel(const DasArgs a) {
         for (uint32_t j = written; j < a.n_ch; j++) out[static_cast<uint64_t>(j) * a.n_depths + t.r] = clamp_cast<OutT>(0.0f);
 }
 
+// rxDelayDTSPACE3DKernel + sampleDelay3D (RawDelay.cu:126-220, 315-402): for
+// the highest-weight transmit, the delayed, apodised sample times n/W of
+// aperture element (ex, ey) goes to slot (ex - first.x) + (ey - first.y) *
+// layout.x of out[(s*N_ch + slot)*N_d + r].  Every slot is written once per
+// pixel: slots no aperture element maps to, elements outside the F-number
+// aperture and delays out of range give 0.  Lanes = depths; the slots are
+// staged in LDS 128 at a time.
+template <typename RFT, typename OutT>
+__global__ __launch_bounds__(kThreads) void rawdelay3d_kernel(const DasArgs a) {
+    __shared__ float s_lut[kWindowEntries];
+    __shared__ float s_xe[kChunk3D];
+    __shared__ float s_ze[kChunk3D];
+    __shared__ uint32_t s_off[kChunk3D];      // channel offset; kNotConnected: no element in this slot
+
+    const Tile t = decode_tile(a);
+    for (int i = threadIdx.x; i < kWindowEntries; i += kThreads) s_lut[i] = a.lut[i];
+    const ScanView sc = scan_view(a.scans, t.s);
+    const float d = t.valid ? a.depths[t.r] : 0.0f;
+    const float t_ap = (((d * a.dt) * a.c) / (2.0f * a.F)) / a.c / a.dt;
+    const float aDT = t_ap * t_ap;
+    const float sq = sqrtf(aDT);
+    const float invx = 1.0f / fminf(sq, sc.max_ed_x());
+    const float invz = 1.0f / fminf(sq, sc.max_ed_y());
+    const float x = sc.x(), z = sc.z();
+    const float px = x + sc.dir_x() * d, pz = z + sc.dir_z() * d;
+    const float py = sc.dir_y() * d;
+    const float py2 = py * py;
+    OutT* out = static_cast<OutT*>(a.out) + static_cast<uint64_t>(t.s) * a.n_ch * a.n_depths;
+    const int k = sc.sel();
+    if (k < 0) {
+        __syncthreads();
+        if (t.valid)
+            for (uint32_t j = 0; j < a.n_ch; j++) out[static_cast<uint64_t>(j) * a.n_depths + t.r] = clamp_cast<OutT>(0.0f);
+        return;
+    }
+    const TxDT tx = sc.tx(k);
+    const uint32_t fx = tx.first_x, fy = tx.first_y;
+    const uint32_t wx = tx.last_x > fx ? tx.last_x - fx : 0u, wy = tx.last_y > fy ? tx.last_y - fy : 0u;
+    const uint32_t lx = a.layout_x;
+    const RFT* rf_tx = static_cast<const RFT*>(a.rf) + static_cast<uint64_t>(tx.tx_idx) * a.n_ch * a.n_s;
+    const __amdgpu_buffer_rsrc_t rsrc = rf_resource(rf_tx, a.n_ch * a.n_s * sizeof(RFT));
+    auto weight = [&](float xe, float ze, bool& member) {
+        const float dxs = xe - x, dzs = ze - z;
+        member = (dxs * dxs + dzs * dzs) <= aDT;
+        return sqrtf(s_lut[window_index(dxs * invx)] * s_lut[window_index(dzs * invz)]);
+    };
+    // pass 1: n and W over the aperture, elements in x-outer / y-inner order
+    float wsum = 0.0f;
+    int nadd = 0;
+    const uint32_t n_ap = wx * wy;
+    for (uint32_t base = 0; base < n_ap; base += kChunk3D) {
+        const uint32_t cnt = min(static_cast<uint32_t>(kChunk3D), n_ap - base);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < cnt; j += kThreads) {
+            const uint32_t q = base + j;
+            const uint32_t e = (fx + q / wy) + (fy + q % wy) * lx;
+            s_xe[j] = a.ex[e];
+            s_ze[j] = a.ey[e];
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; j++) {
+            bool m;
+            const float w = weight(s_xe[j], s_ze[j], m);
+            if (m) {
+                wsum += w;
+                nadd++;
+            }
+        }
+    }
+    float scale = 1.0f;
+    if (nadd > 0) scale = 1.0f / wsum * static_cast<float>(nadd);
+    // pass 2: every output slot once
+    for (uint32_t base = 0; base < a.n_ch; base += kChunk3D) {
+        const uint32_t cnt = min(static_cast<uint32_t>(kChunk3D), a.n_ch - base);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < cnt; j += kThreads) {
+            const uint32_t slot = base + j;
+            const uint32_t ox = slot % lx, oy = slot / lx;
+            if (ox < wx && oy < wy) {
+                const uint32_t e = (fx + ox) + (fy + oy) * lx;
+                s_xe[j] = a.ex[e];
+                s_ze[j] = a.ey[e];
+                s_off[j] = (e % a.n_ch) * a.n_s;
+            } else {
+                s_off[j] = kNotConnected;
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; j++) {
+            float sample = 0.0f;
+            const uint32_t off = s_off[j];
+            if (off != kNotConnected) {
+                bool m;
+                const float w = weight(s_xe[j], s_ze[j], m);
+                if (m) {
+                    const float ax = s_xe[j] - px, az = s_ze[j] - pz;
+                    const float tau = tx.t0 + (sqrt_rn((ax * ax + az * az) + py2) + d);
+                    gather<RFT, true>(rsrc, off, a.n_s, tau, w, sample);
+                }
+            }
+            if (t.valid) out[static_cast<uint64_t>(base + j) * a.n_depths + t.r] = clamp_cast<OutT>(sample * scale);
+        }
+    }
+}
+
 // ---- persistent work queue over many 2D frames -----------------------------
 
 template <typename RFT, typename OutT>
@@ -619,8 +902,20 @@ static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool i
             else hipLaunchKernelGGL((KERNEL<RFT, OutT, false, false, false>), grid, block, 0, stream, a);           \
         }                                                                                                           \
     } while (0)
-    if (is3d) SUPRA_DAS_DISPATCH(das3d_kernel);
-    else SUPRA_DAS_DISPATCH(das2d_kernel);
+    if (is3d) {
+        SUPRA_DAS_DISPATCH(das3d_kernel);
+    } else if (!testsig && !interp_tx && a.n_frames >= 2 && a.scan_frame_stride == 0 && a.depth_frame_stride == 0 &&
+               a.el_frame_stride == 0 && a.frame_c == nullptr && a.norm_out == nullptr) {
+        // frames sharing one geometry: kFramesPerWave frames per wave
+        constexpr int FB = kFramesPerWave;
+        const uint64_t units = static_cast<uint64_t>((a.n_depths + kWaveDepths - 1) / kWaveDepths) * a.n_rx *
+                               ((a.n_frames + FB - 1) / FB);
+        dim3 g2(static_cast<uint32_t>((units + kWavesPerGroup - 1) / kWavesPerGroup));
+        if (map) hipLaunchKernelGGL((das2d_frames_kernel<RFT, OutT, FB, true>), g2, block, 0, stream, a);
+        else hipLaunchKernelGGL((das2d_frames_kernel<RFT, OutT, FB, false>), g2, block, 0, stream, a);
+    } else {
+        SUPRA_DAS_DISPATCH(das2d_kernel);
+    }
 #undef SUPRA_DAS_DISPATCH
     return hipGetLastError();
 }
@@ -632,9 +927,19 @@ hipError_t launch_das_batch(const DasBatchArgs& b, uint32_t workgroups, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, hipStream_t stream) {
+hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, hipStream_t stream) {
     const uint64_t blocks = static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames;
     dim3 grid(static_cast<uint32_t>(blocks)), block(kThreads);
+    if (is3d) {
+        if (rf_dtype == SUPRA_INT16) {
+            if (out_dtype == SUPRA_FLOAT32) hipLaunchKernelGGL((rawdelay3d_kernel<int16_t, float>), grid, block, 0, stream, a);
+            else hipLaunchKernelGGL((rawdelay3d_kernel<int16_t, int16_t>), grid, block, 0, stream, a);
+        } else {
+            if (out_dtype == SUPRA_FLOAT32) hipLaunchKernelGGL((rawdelay3d_kernel<float, float>), grid, block, 0, stream, a);
+            else hipLaunchKernelGGL((rawdelay3d_kernel<float, int16_t>), grid, block, 0, stream, a);
+        }
+        return hipGetLastError();
+    }
     if (rf_dtype == SUPRA_INT16) {
         if (out_dtype == SUPRA_FLOAT32) hipLaunchKernelGGL((rawdelay2d_kernel<int16_t, float>), grid, block, 0, stream, a);
         else hipLaunchKernelGGL((rawdelay2d_kernel<int16_t, int16_t>), grid, block, 0, stream, a);

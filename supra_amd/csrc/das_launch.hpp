@@ -14,8 +14,13 @@ uint32_t das_depth_tile();
 hipError_t launch_das(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, bool testsig, bool interp_tx,
                       bool map, hipStream_t stream);
 
-// Launch the 2D pre-delay kernel (output [frame][scanline][channel][depth]).
-hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, hipStream_t stream);
+// Launch the DelayAndStdDev / CoherenceFactorDelayAndSum kernels (kind
+// SUPRA_BF_DELAY_AND_STD_DEV or SUPRA_BF_COHERENCE_FACTOR, das_variants.hip).
+hipError_t launch_das_variant(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, int kind, bool interp_tx,
+                              hipStream_t stream);
+
+// Launch the 2D or 3D pre-delay kernel (output [frame][scanline][channel][depth]).
+hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, bool is3d, hipStream_t stream);
 
 // Persistent work-queue DAS over the frames of a batch (das_kernels.hip).
 hipError_t launch_das_batch(const DasBatchArgs& b, uint32_t workgroups, hipStream_t stream);

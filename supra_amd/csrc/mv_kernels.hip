@@ -21,10 +21,12 @@
 //   3. one wave (L <= 64) or half-wave (L <= 32) per output sample: row i of
 //      R~ = sum over |tau - t| <= T (:197-228) in lane i's registers,
 //      diagonal loading alpha*trace/L_t (:239-305), a Cholesky factorisation
-//      with FP64 pivots and diagonal, triangular solves for w = R~^-1 1, an
-//      FP64 residual check with up to two steps of iterative refinement
-//      (FP32 factors, FP64 residual) for ill-conditioned samples, and the
-//      output y = sum_i (sum_k x[f+k+i]) w_i / (sum w * K^p) (:307-378).
+//      with FP64 pivots and diagonal, triangular solves for w = R~^-1 1, and,
+//      for samples that may be ill-conditioned (condition bound 1 + L_t/alpha
+//      of the loaded system and squared-pivot ratio both large), an FP64
+//      residual check with up to two steps of iterative refinement (FP32
+//      factors, FP64 residual); then the output
+//      y = sum_i (sum_k x[f+k+i]) w_i / (sum w * K^p) (:307-378).
 //
 // Numerics are FP32 with FP64 only where stated; results are compared with
 // the FP64 host oracle (per-sample relative error), not bit for bit: the
@@ -168,6 +170,21 @@ __device__ inline double sub_sum(double v) {
     for (int m = 32 / PPW; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
     return v;
 }
+template <int PPW>
+__device__ inline float sub_max(float v) {
+    for (int m = 32 / PPW; m > 0; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+template <int PPW>
+__device__ inline float sub_min(float v) {
+    for (int m = 32 / PPW; m > 0; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Conditioning below which the FP64 residual check is skipped: the a-priori
+// bound 1 + n/alpha of a loaded system, or the squared-pivot ratio.
+constexpr float kKappaSafe = 1.0e4f;
+constexpr float kPivotRatioSafe = 64.0f;
 
 // Solve (G G^T) v = b for the sub-wave's pixel: G's row i is in lane i's
 // registers a[] (strictly lower part), ginv is lane i's 1/G[i][i]; the
@@ -570,8 +587,20 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
         const bool ok = (PPW == 1) ? nonpos == 0 : ((nonpos >> (32 * half)) & 0xffffffffull) == 0;
         wave_sync();
         float w = chol_solve<LMAX, PPW>(row, ginv, 1.0f, n, li, half, lmat);
-        // FP64 residual r = R~ w - 1, up to two refinement steps
-        for (int it = 0; it < 3; it++) {
+        // FP64 residual r = R~ w - 1 and up to two refinement steps, only for
+        // systems that may be ill-conditioned.  R~ is a sum of outer products
+        // (positive semi-definite), so with the loading delta = alpha trace/n
+        // its condition number is at most (trace + delta) / delta = 1 + n/alpha;
+        // the ratio of the largest to the smallest squared pivot is a lower
+        // bound on it.  Below kKappaSafe (bound) or kPivotRatioSafe (pivots)
+        // the FP32 factorisation already solves to about kappa * 6e-8.
+        float dmax = (li < n) ? static_cast<float>(dd) : 0.0f;
+        float dmin = (li < n) ? static_cast<float>(dd) : FLT_MAX;
+        dmax = sub_max<PPW>(dmax);
+        dmin = sub_min<PPW>(dmin);
+        const bool loaded_ok = a.alpha > 0.0f && 1.0f + static_cast<float>(n) / a.alpha <= kKappaSafe;
+        const bool suspect = ok && !loaded_ok && !(dmax <= kPivotRatioSafe * dmin);
+        for (int it = 0; it < 3 && __any(suspect); it++) {
             double r = 0.0;
 #pragma unroll
             for (int k = 0; k < LMAX; k++) {
@@ -583,7 +612,7 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
             // refine only where single precision left a relative residual
             // above 1e-3 (ill-conditioned systems); both halves of a wave
             // take the same number of steps
-            const bool more = (rn > 1e-6 * n) && ok;
+            const bool more = (rn > 1e-6 * n) && suspect;
             const bool any_more = __any(more);
             if (!any_more || it == 2) break;
             const float dw = chol_solve<LMAX, PPW>(row, ginv, static_cast<float>(r), n, li, half, lmat);

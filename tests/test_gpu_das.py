@@ -144,6 +144,33 @@ def test_frames_batch_matches_single(device):
         assert_bit_exact(batch[f], O.das(g, rfs[f], w.fs_hz, **kw), f"frame {f}")
 
 
+@pytest.mark.parametrize("n_frames,rf_dt,out_dt,sparse", [(3, "i16", "f32", False), (8, "i16", "i16", False),
+                                                        (11, "f32", "f32", False), (9, "i16", "f32", True)])
+def test_frames_per_wave_bit_exact(device, n_frames, rf_dt, out_dt, sparse):
+    """Frames that share a geometry run several per wave (one delay
+    evaluation per element and pixel for all of them): full and partial frame
+    groups, int16 and float RF and images, and a sparse element map are bit
+    for bit the oracle's, frame by frame."""
+    w = W.linear_2d(num_scatterers=15)
+    g = w.geometry()
+    if sparse:
+        emap = (np.arange(128) % 64).astype(np.int32)
+        emap[np.random.default_rng(3).choice(128, 17, replace=False)] = -1
+        g.element_to_channel = emap
+    dt = np.int16 if rf_dt == "i16" else np.float32
+    rfs = np.stack([W.linear_2d(num_scatterers=15, scatter_seed=40 + f, noise_seed=60 + f).rf_host(dtype=dt)
+                    for f in range(n_frames)])
+    kw = w.das_kwargs()
+    odt = torch.int16 if out_dt == "i16" else None
+    batch = gpu_das(g, rfs, w.fs_hz, out_dtype=odt, **kw)
+    assert batch.shape == (n_frames, 2080, 128)
+    for f in range(n_frames):
+        ref = O.das(g, rfs[f], w.fs_hz, out_int16=(out_dt == "i16"), **kw)
+        if out_dt == "i16":
+            ref = ref.astype(np.int16)
+        assert_bit_exact(batch[f], ref, f"frame {f} of {n_frames}")
+
+
 def test_node_dispatch_and_xml(device):
     xml = """<node type="BeamformingNode" id="BEAM">
       <param name="fNumber" type="double">1</param>

@@ -44,6 +44,39 @@ def test_rawdelay_bit_exact(device):
     rd.close()
 
 
+@pytest.mark.parametrize("n_el,odt", [(6, torch.float32), (8, torch.int16)])
+def test_rawdelay3d_bit_exact(device, n_el, odt):
+    """3D pre-delay of a matrix probe (RawDelay.cu:126-220, 315-402): every
+    slot of every pixel bit for bit the FP32 oracle's, then minimum variance
+    on the pre-delayed volume against the FP64 oracle."""
+    w = W.matrix_3d_small(n_el=n_el, n_lines=5, n_depths=150, n_samples=700)
+    g = w.geometry()
+    rf = w.rf_host()
+    rd = RawDelay(g)
+    res = rd.perform_delay(USRawData(torch.from_numpy(rf).cuda(), w.fs_hz, g), 1.0, 1.54e6, N.WINDOW_HANN, 0.0,
+                           out_dtype=odt)
+    torch.cuda.synchronize()
+    gpu = res.data.cpu().numpy()
+    ref = O.rawdelay(g, rf, w.fs_hz, 1.0, 1.54e6, N.WINDOW_HANN, out_int16=(odt == torch.int16))
+    if odt == torch.int16:
+        ref = ref.astype(np.int16)
+    assert gpu.shape == (g.num_rx, n_el * n_el, g.num_depths)
+    same = (gpu == ref) | (np.isnan(gpu) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} pre-delayed samples differ"
+    if odt == torch.float32:
+        L = n_el * n_el // 2
+        img = mv_perform_rx_beamforming(res, L, 2, 1.5, False, 0.05)
+        torch.cuda.synchronize()
+        out = img.data.cpu().numpy()
+        mref = O.mv(gpu, L, 2, 1.5, False, 0.05)
+        huge = (np.abs(gpu) > 1e19).any(axis=1)
+        keep = ~np.array([np.convolve(h, np.ones(5), "same") > 0 for h in huge]).T
+        err = rel_errors(out[keep], mref[keep])
+        print(f"MV on the 3D pre-delayed volume: max {err.max():.3e} p99 {np.quantile(err, 0.99):.3e}")
+        assert np.quantile(err, 0.99) < 1e-4 and err.max() < 1e-2
+    rd.close()
+
+
 def test_rawdelay_node(device):
     w = W.linear_2d(num_scatterers=30)
     g = w.geometry()
@@ -64,6 +97,25 @@ def test_mv_plane_wave_closed_form(device, alpha, T, p):
     out = gpu_mv(x, sub_array_size=32, temporal_smoothing=T, sub_array_scaling_power=p, loading_factor=alpha)
     expected = (n_ch * s[:, 0, :].astype(np.float64) * 33 ** (1 - p)).T
     assert np.abs(out / expected - 1).max() < 1e-4
+
+
+@pytest.mark.parametrize("alpha,T,p", [(0.01, 5, 1.5), (1.0, 2, 1.0)])
+def test_mv_means_channel_ramp_closed_form(device, alpha, T, p):
+    """Mean removal on x[c, t] = a(t) + b(t) c (exact in float32): R~ is
+    rank one along 1 and y = K^(1-p) (a + b ((K-1)/2 + (L-1)/2)), times N_ch,
+    independent of alpha and T."""
+    rng = np.random.default_rng(13)
+    n_rx, n_ch, n_d, L = 8, 64, 200, 32
+    K = n_ch - L + 1
+    a = rng.integers(200, 300, (n_rx, n_d)).astype(np.float64)
+    b = rng.integers(4, 9, (n_rx, n_d)) / 4.0 * rng.choice([-1.0, 1.0], (n_rx, n_d))
+    x = (a[:, None, :] + b[:, None, :] * np.arange(n_ch)[None, :, None]).astype(np.float32)
+    out = gpu_mv(x, sub_array_size=L, temporal_smoothing=T, sub_array_scaling_power=p, compute_means=True,
+                 loading_factor=alpha)
+    expected = (n_ch * K ** (1 - p) * (a + b * ((K - 1) / 2 + (L - 1) / 2))).T
+    err = np.abs(out / expected - 1).max()
+    print(f"MV means ramp closed form alpha={alpha} T={T}: max rel err {err:.2e}")
+    assert err < 1e-4
 
 
 def test_mv_predelayed_frame_vs_fp64_oracle(device):

@@ -240,3 +240,34 @@ def test_mixed_precision_mode():
     rmx, cmx, _ = adj64(w, g, rf, 1520e3, gi, precision=6432)
     assert np.linalg.norm(r32 - rmx) <= 1e-3 * np.linalg.norm(rmx)
     assert abs(c32[0] - cmx[0]) <= 1e-4 * abs(cmx[0])
+
+
+def test_decision_diff_classifier():
+    """das_decision_diff marks the pixels where FP32 and FP64 take different
+    discrete decisions: a rectangular window never sets the window flag, and
+    outside the marked pixels the FP32 and FP64 images agree to
+    single-precision rounding of the delays, while the marked pixels hold the
+    large differences."""
+    w = W.linear_2d(num_scatterers=60)
+    g = w.geometry()
+    rf = w.rf_host()
+    kw = w.das_kwargs()
+    rect = O.das_decision_diff(g, w.num_samples, w.fs_hz, 1.0, 1540e3, N.WINDOW_RECTANGULAR)
+    assert not (rect & 2).any()
+    fl = O.das_decision_diff(g, w.num_samples, w.fs_hz, 1.0, 1540e3, N.WINDOW_HANN)
+    assert (fl & 2).any()
+    y32 = O.das(g, rf, w.fs_hz, **kw).astype(np.float64)
+    y64 = O.das(g, rf, w.fs_hz, precision=64, **kw)
+    ok = (np.abs(y32) < 1e30) & (np.abs(y64) < 1e30)
+    scale = np.abs(y64[ok]).max()
+    clean = ok & (fl == 0)
+    tied = ok & (fl != 0)
+    assert np.abs(y32[clean] - y64[clean]).max() <= 1e-4 * scale
+    assert np.abs(y32[tied] - y64[tied]).max() >= 1e-3 * scale
+    same = O.das_decision_diff(g, w.num_samples, w.fs_hz, 1.0, 1540e3, N.WINDOW_HANN, precision_b=32)
+    assert not same.any()
+    # across speeds of sound only the delay decisions (floor, range) move in
+    # exact arithmetic; membership and window changes are rounding ties
+    moved = O.das_decision_diff(g, w.num_samples, w.fs_hz, 1.0, 1540e3, N.WINDOW_HANN, precision_a=64,
+                                speed_of_sound_b_mm_per_s=1545e3)
+    assert (moved & 4).mean() > 0.5 and (moved & 3).mean() < 0.2

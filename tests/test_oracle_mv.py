@@ -24,6 +24,35 @@ def test_rawdelay_sums_to_das():
         assert np.all(rd[:, 63, :] == 0)            # only last-first = 63 channels are written
 
 
+def test_rawdelay3d_sums_to_das3d():
+    """3D pre-delay: summing the slots of each pixel gives the 3D DAS volume
+    (same F-number and window, no offset); slot (ex - first.x) + (ey -
+    first.y) * layout.x carries element (ex, ey), other slots stay 0."""
+    w = W.matrix_3d_small(n_el=6, n_lines=5, n_depths=120, n_samples=700)
+    g = w.geometry()
+    rf = w.rf_host()
+    for win in (N.WINDOW_HANN, N.WINDOW_RECTANGULAR):
+        kw = dict(w.das_kwargs(), window_type=win)
+        rd = O.rawdelay(g, rf, w.fs_hz, kw["f_number"], kw["speed_of_sound_mm_per_s"], win, precision=64)
+        das = O.das(g, rf, w.fs_hz, precision=64, **kw)
+        assert rd.shape == (g.num_rx, rf.shape[1], g.num_depths)
+        s = rd.sum(1).T
+        fin = np.isfinite(das) & (np.abs(das) < 1e30)
+        assert np.abs(s[fin] - das[fin]).max() <= 1e-11 * np.abs(das[fin]).max()
+        first = g.scanlines["tx"][:, 0]["first"]
+        last = g.scanlines["tx"][:, 0]["last"]
+        lx = g.element_layout[0]
+        for sl in range(g.num_rx):
+            used = {(ex - first[sl, 0]) + (ey - first[sl, 1]) * lx
+                    for ex in range(first[sl, 0], last[sl, 0]) for ey in range(first[sl, 1], last[sl, 1])}
+            unused = sorted(set(range(rf.shape[1])) - used)
+            assert np.all(rd[sl, unused, :] == 0)
+    rf_c = np.full(w.rf_shape, 10, dtype=np.int16)
+    rd = O.rawdelay(g, rf_c, w.fs_hz, 1.0, 1.54e6, N.WINDOW_RECTANGULAR, precision=64)
+    assert set(np.unique(np.round(rd[:, :, 60:100], 9))) <= {0.0, 10.0}
+    assert (np.round(rd[:, :, 60:100], 9) == 10.0).any()
+
+
 def test_rawdelay_constant_rf_channel_weights():
     """Constant RF k: channel j carries w_j * k * n / W, so with a rectangular
     window every in-aperture channel equals k * n / n = k."""
@@ -81,13 +110,15 @@ def test_mv_infinite_loading_closed_form():
             assert abs(out[t, s] - n_ch * y) <= 1e-6 * max(1.0, abs(n_ch * y)), (s, t)
 
 
-def test_mv_system_matches_numpy():
-    """The assembled R~ + delta I equals the definition evaluated in numpy, and
-    the oracle's weights solve it."""
+@pytest.mark.parametrize("means", [False, True])
+def test_mv_system_matches_numpy(means):
+    """The assembled R~ + delta I equals the definition evaluated in numpy
+    (with the per-sample subarray mean removed when means is on), and the
+    oracle's weights solve it."""
     x, f, l = random_apertures(seed=3)
     L, T, alpha = 32, 3, 0.05
     for (s, t) in [(0, 10), (1, 0), (2, 59), (1, 31)]:
-        _, A, wts, meta = O.mv(x, L, T, 1.5, False, alpha, debug=(s, t))
+        _, A, wts, meta = O.mv(x, L, T, 1.5, means, alpha, debug=(s, t))
         first, last, lt, kact = meta
         n_d = x.shape[2]
         R = np.zeros((L, L))
@@ -95,14 +126,38 @@ def test_mv_system_matches_numpy():
             f_, l_ = f[s, tau], l[s, tau]
             lt_ = min(L, l_ - f_ + 1)
             k_ = min(x.shape[1] - L + 1, l_ - f_ - lt_ + 2)
-            for k in range(k_):
-                v = x[s, f_ + k: f_ + k + lt_, tau].astype(np.float64)
+            subs = np.stack([x[s, f_ + k: f_ + k + lt_, tau].astype(np.float64) for k in range(k_)])
+            if means:
+                subs = subs - subs.mean(axis=0)
+            for v in subs:
                 R[:lt_, :lt_] += np.outer(v, v)
         Rt = R[:lt, :lt]
         Rt = Rt + alpha * np.trace(Rt) / lt * np.eye(lt)
         assert np.allclose(A, Rt, rtol=1e-12, atol=1e-12)
         assert np.allclose(np.linalg.solve(A, np.ones(lt)), wts, rtol=1e-10)
         assert (first, last) == (f[s, t], l[s, t])
+
+
+@pytest.mark.parametrize("alpha,T,p", [(0.01, 5, 1.5), (1.0, 0, 1.0), (2.0, 3, 1.5)])
+def test_mv_means_channel_ramp_closed_form(alpha, T, p):
+    """Mean removal on x[c, t] = a(t) + b(t) c: every subarray minus the
+    subarray mean is b (k - (K-1)/2) 1, so R~ = S 1 1^T (S > 0) and, like the
+    plane wave, w = 1 / (S (L + alpha)); the output is the mean-free
+    beamformer of the subarray sums: y = K^(1-p) (a + b (f + (K-1)/2 +
+    (L-1)/2)) with f = 0, times N_ch — independent of alpha and T.  A sign
+    slip in the mean removal or a mean taken over the wrong index would
+    leave a rank-2 R~ and break it."""
+    rng = np.random.default_rng(12)
+    n_rx, n_ch, n_d, L = 3, 64, 90, 32
+    K = n_ch - L + 1
+    # integers and quarters: the ramp is exact in float32
+    a = rng.integers(200, 300, (n_rx, n_d)).astype(np.float64)
+    b = rng.integers(4, 9, (n_rx, n_d)) / 4.0 * rng.choice([-1.0, 1.0], (n_rx, n_d))
+    x = (a[:, None, :] + b[:, None, :] * np.arange(n_ch)[None, :, None]).astype(np.float32)
+    assert np.array_equal(x.astype(np.float64), a[:, None, :] + b[:, None, :] * np.arange(n_ch)[None, :, None])
+    out = O.mv(x, L, T, p, True, alpha)
+    expected = (n_ch * K ** (1 - p) * (a + b * ((K - 1) / 2 + (L - 1) / 2))).T
+    assert np.allclose(out, expected, rtol=1e-9), np.abs(out / expected - 1).max()
 
 
 def test_mv_on_predelayed_frame_is_finite():
