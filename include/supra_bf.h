@@ -209,6 +209,30 @@ int supra_das_forward_c(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf,
                         int32_t interpolate_transmits, int32_t additional_offset,
                         float* out, float* norm_out, void* stream);
 
+/* Largest device scratch of supra_das_forward_c_noalloc, in bytes: the
+ * per-frame tables of one call, frames * (num_rx_scanlines * 192 +
+ * num_depths * 4 + num_elements * 4 + 4) bytes rounded up per part. */
+#define SUPRA_NOALLOC_SCRATCH_BYTES (8u << 20)
+
+/* supra_das_forward_c without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as supra_das_forward_c; the per-frame dt-space
+ * tables live in a file-scope device array of SUPRA_NOALLOC_SCRATCH_BYTES
+ * per GPU (looked up at supra_bf_create, held until the process ends), and
+ * calls whose tables need more return SUPRA_ERR_TOO_LARGE.  Every context on
+ * one GPU shares that array: calls that overlap on one GPU (different
+ * streams or threads without ordering) race on it; issue them in order on
+ * one stream.  The context must be 2D (its device geometry is uploaded at
+ * supra_bf_create). */
+int supra_das_forward_c_noalloc(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf,
+                                uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                                double sampling_frequency_hz, double f_number,
+                                const double* speed_of_sound_mm_per_s,
+                                int32_t window_type, float window_parameter,
+                                int32_t interpolate_transmits, int32_t additional_offset,
+                                float* out, float* norm_out, void* stream);
+
 /* Adjoint of supra_das_forward_c.  No reference counterpart.  Given
  * grad_image = dL/d(out) (device [frame][depth][scanline]) and the forward's
  * norm_out, writes (each may be NULL):

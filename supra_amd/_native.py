@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libsupra_bf.so"
 # status codes / enums (supra_bf.h)
 SUPRA_OK = 0
 SUPRA_ERR_INVALID_ARG, SUPRA_ERR_UNSUPPORTED, SUPRA_ERR_HIP, SUPRA_ERR_NO_DEVICE, SUPRA_ERR_TOO_LARGE = -1, -2, -3, -4, -5
+NOALLOC_SCRATCH_BYTES = 8 << 20     # SUPRA_NOALLOC_SCRATCH_BYTES
 SUPRA_INT16 = 1
 SUPRA_FLOAT32 = 2
 SUPRA_UINT8 = 3
@@ -142,6 +143,9 @@ _SIGNATURES = {
     "supra_das_forward_c": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
                                       C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "supra_das_forward_c_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,
+                                              C.c_uint32, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_float,
+                                              C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "supra_das_adjoint": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
                                     C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

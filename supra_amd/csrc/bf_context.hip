@@ -139,6 +139,36 @@ int check_device(const supra_bf_ctx* ctx) {
 
 }  // namespace
 
+// Device scratch of supra_das_forward_c_noalloc: the per-frame dt-space
+// tables are built in this file-scope array (one copy per GPU, held until
+// the process ends) instead of in memory the library allocates.  Host code
+// never uses the array's name as an address: supra_bf_create looks up each
+// device's copy once, with that device current, and keeps the pointer.
+static __device__ __align__(256) unsigned char scratch[SUPRA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+int ensure_mm_geometry(supra_bf_ctx* ctx);
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;   // per device; nullptr until looked up
+
+int noalloc_scratch(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return fail(SUPRA_ERR_NO_DEVICE, "scratch: invalid device %d", device);
+    if (static_cast<size_t>(device) >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "scratch: hipGetSymbolAddress");
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return SUPRA_OK;
+}
+
+}  // namespace
+
 extern "C" int supra_bf_create(const supra_rx_geometry* g, supra_bf_ctx** out_ctx) {
     clear_error();
     if (!g || !out_ctx) return fail(SUPRA_ERR_INVALID_ARG, "create: null argument");
@@ -224,6 +254,14 @@ extern "C" int supra_bf_create(const supra_rx_geometry* g, supra_bf_ctx** out_ct
             delete ctx;
             return hip_fail(e, "create: element map upload");
         }
+    }
+    // The differentiable path's device geometry is set up here, so that its
+    // allocation-free entry point allocates nothing on any call.
+    int st = ctx->is3d ? SUPRA_OK : ensure_mm_geometry(ctx);
+    if (st == SUPRA_OK) st = noalloc_scratch(ctx->device, &ctx->noalloc_scratch);
+    if (st != SUPRA_OK) {
+        delete ctx;
+        return st;
     }
     *out_ctx = ctx;
     return SUPRA_OK;
@@ -542,17 +580,22 @@ struct FrameTables {
     float* c;
 };
 
-// dt-space tables of every frame for the device array c (no host round trip).
-int build_frame_tables(supra_bf_ctx* ctx, uint32_t frames, const double* c_dev, double dt, hipStream_t stream,
-                       FrameTables* out) {
+// Bytes of the per-frame tables of `frames` frames (256-byte aligned parts).
+size_t frame_table_bytes(const supra_bf_ctx* ctx, uint32_t frames) {
+    return align256(static_cast<size_t>(frames) * ctx->n_rx * sizeof(ScanlineDT)) +
+           align256(static_cast<size_t>(frames) * ctx->n_depths * sizeof(float)) +
+           align256(static_cast<size_t>(frames) * ctx->n_el * sizeof(float)) +
+           align256(static_cast<size_t>(frames) * sizeof(float));
+}
+
+// dt-space tables of every frame for the device array c (no host round trip),
+// built at `base` (frame_table_bytes(ctx, frames) bytes of device memory).
+int build_frame_tables_at(supra_bf_ctx* ctx, void* base_ptr, uint32_t frames, const double* c_dev, double dt,
+                          hipStream_t stream, FrameTables* out) {
     const size_t b_scans = align256(static_cast<size_t>(frames) * ctx->n_rx * sizeof(ScanlineDT));
     const size_t b_depths = align256(static_cast<size_t>(frames) * ctx->n_depths * sizeof(float));
     const size_t b_ex = align256(static_cast<size_t>(frames) * ctx->n_el * sizeof(float));
-    const size_t b_c = align256(static_cast<size_t>(frames) * sizeof(float));
-    int st = ensure_capacity(&ctx->d_frame_tables, &ctx->frame_tables_cap, b_scans + b_depths + b_ex + b_c,
-                             "differentiable DAS: table memory");
-    if (st != SUPRA_OK) return st;
-    char* base = static_cast<char*>(ctx->d_frame_tables);
+    char* base = static_cast<char*>(base_ptr);
     out->scans = reinterpret_cast<ScanlineDT*>(base);
     out->depths = reinterpret_cast<float*>(base + b_scans);
     out->ex = reinterpret_cast<float*>(base + b_scans + b_depths);
@@ -577,6 +620,15 @@ int build_frame_tables(supra_bf_ctx* ctx, uint32_t frames, const double* c_dev, 
     hipError_t e = launch_frame_tables(a, stream);
     if (e != hipSuccess) return hip_fail(e, "differentiable DAS: table launch");
     return SUPRA_OK;
+}
+
+// The same in the context's own table memory, grown on demand.
+int build_frame_tables(supra_bf_ctx* ctx, uint32_t frames, const double* c_dev, double dt, hipStream_t stream,
+                       FrameTables* out) {
+    int st = ensure_capacity(&ctx->d_frame_tables, &ctx->frame_tables_cap, frame_table_bytes(ctx, frames),
+                             "differentiable DAS: table memory");
+    if (st != SUPRA_OK) return st;
+    return build_frame_tables_at(ctx, ctx->d_frame_tables, frames, c_dev, dt, stream, out);
 }
 
 // Which (transmit, channel) RF lines the adjoint writes and how often: with
@@ -668,29 +720,42 @@ int check_diff_call(const supra_bf_ctx* ctx, const char* what, uint32_t num_fram
 
 }  // namespace
 
-extern "C" int supra_das_forward_c(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf, uint32_t num_tx,
-                                   uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
-                                   double f_number, const double* speed_of_sound_mm_per_s, int32_t window_type,
-                                   float window_parameter, int32_t interpolate_transmits, int32_t additional_offset,
-                                   float* out, float* norm_out, void* stream) {
-    clear_error();
-    if (!ctx || !rf || !out || !speed_of_sound_mm_per_s)
-        return fail(SUPRA_ERR_INVALID_ARG, "das forward: null argument");
+namespace {
+
+// supra_das_forward_c with its per-frame tables either in the context's own
+// memory (noalloc == false) or in this device's file-scope scratch array.
+int das_forward_c(supra_bf_ctx* ctx, bool noalloc, uint32_t num_frames, const float* rf, uint32_t num_tx,
+                  uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz, double f_number,
+                  const double* speed_of_sound_mm_per_s, int32_t window_type, float window_parameter,
+                  int32_t interpolate_transmits, int32_t additional_offset, float* out, float* norm_out,
+                  void* stream) {
+    const char* what = noalloc ? "das forward (no allocation)" : "das forward";
+    if (!ctx || !rf || !out || !speed_of_sound_mm_per_s) return fail(SUPRA_ERR_INVALID_ARG, "%s: null argument", what);
     if (num_frames == 0) return SUPRA_OK;
-    int st = check_diff_call(ctx, "das forward", num_frames, num_tx, num_channels, num_samples,
-                             sampling_frequency_hz, f_number, window_type, interpolate_transmits);
+    int st = check_diff_call(ctx, what, num_frames, num_tx, num_channels, num_samples, sampling_frequency_hz,
+                             f_number, window_type, interpolate_transmits);
     if (st != SUPRA_OK) return st;
     const uint64_t tile = das_depth_tile();
     const uint64_t blocks = ((ctx->n_depths + tile - 1) / tile) * ctx->n_rx * static_cast<uint64_t>(num_frames);
-    if (blocks >= (1ull << 31)) return fail(SUPRA_ERR_TOO_LARGE, "das forward: too many frames for one launch");
+    if (blocks >= (1ull << 31)) return fail(SUPRA_ERR_TOO_LARGE, "%s: too many frames for one launch", what);
+    if (noalloc && frame_table_bytes(ctx, num_frames) > SUPRA_NOALLOC_SCRATCH_BYTES)
+        return fail(SUPRA_ERR_TOO_LARGE, "%s: the tables of %u frames need %zu bytes, more than the %u-byte scratch",
+                    what, num_frames, frame_table_bytes(ctx, num_frames), SUPRA_NOALLOC_SCRATCH_BYTES);
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     if ((st = check_device(ctx)) != SUPRA_OK) return st;
-    if ((st = ensure_mm_geometry(ctx)) != SUPRA_OK) return st;
     hipStream_t hs = static_cast<hipStream_t>(stream);
     const double dt = 1.0 / sampling_frequency_hz;
     FrameTables ft;
-    if ((st = build_frame_tables(ctx, num_frames, speed_of_sound_mm_per_s, dt, hs, &ft)) != SUPRA_OK) return st;
+    if (noalloc) {
+        if (!ctx->d_scan_mm || !ctx->noalloc_scratch)
+            return fail(SUPRA_ERR_INVALID_ARG, "%s: context has no device geometry", what);
+        st = build_frame_tables_at(ctx, ctx->noalloc_scratch, num_frames, speed_of_sound_mm_per_s, dt, hs, &ft);
+    } else {
+        if ((st = ensure_mm_geometry(ctx)) != SUPRA_OK) return st;
+        st = build_frame_tables(ctx, num_frames, speed_of_sound_mm_per_s, dt, hs, &ft);
+    }
+    if (st != SUPRA_OK) return st;
 
     DasArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -728,6 +793,31 @@ extern "C" int supra_das_forward_c(supra_bf_ctx* ctx, uint32_t num_frames, const
                               ctx->d_e2c != nullptr, hs);
     if (e != hipSuccess) return hip_fail(e, "das forward: launch");
     return SUPRA_OK;
+}
+
+}  // namespace
+
+extern "C" int supra_das_forward_c(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf, uint32_t num_tx,
+                                   uint32_t num_channels, uint32_t num_samples, double sampling_frequency_hz,
+                                   double f_number, const double* speed_of_sound_mm_per_s, int32_t window_type,
+                                   float window_parameter, int32_t interpolate_transmits, int32_t additional_offset,
+                                   float* out, float* norm_out, void* stream) {
+    clear_error();
+    return das_forward_c(ctx, false, num_frames, rf, num_tx, num_channels, num_samples, sampling_frequency_hz,
+                         f_number, speed_of_sound_mm_per_s, window_type, window_parameter, interpolate_transmits,
+                         additional_offset, out, norm_out, stream);
+}
+
+extern "C" int supra_das_forward_c_noalloc(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf,
+                                           uint32_t num_tx, uint32_t num_channels, uint32_t num_samples,
+                                           double sampling_frequency_hz, double f_number,
+                                           const double* speed_of_sound_mm_per_s, int32_t window_type,
+                                           float window_parameter, int32_t interpolate_transmits,
+                                           int32_t additional_offset, float* out, float* norm_out, void* stream) {
+    clear_error();
+    return das_forward_c(ctx, true, num_frames, rf, num_tx, num_channels, num_samples, sampling_frequency_hz,
+                         f_number, speed_of_sound_mm_per_s, window_type, window_parameter, interpolate_transmits,
+                         additional_offset, out, norm_out, stream);
 }
 
 extern "C" int supra_das_adjoint(supra_bf_ctx* ctx, uint32_t num_frames, const float* rf, uint32_t num_tx,

@@ -60,6 +60,9 @@ struct supra_bf_ctx {
     uint32_t n_zero_lines = 0;
     uint32_t* d_zero_lines = nullptr;
     uint32_t max_aperture = 0;   // largest last_x - first_x over scanlines and transmits
+    // This device's copy of the file-scope scratch array of the
+    // allocation-free forward (shared by every context on the device; not owned).
+    void* noalloc_scratch = nullptr;
 
     ~supra_bf_ctx() {
         if (tables_used) (void)hipEventDestroy(tables_used);
