@@ -171,6 +171,15 @@ _SIGNATURES = {
     "supra_envelope_logcompress_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                                  C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int32,
                                                  C.c_void_p]),
+    "supra_scanconv_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_double, C.c_uint32, C.c_double,
+                                        C.c_uint32, C.POINTER(C.c_void_p)]),
+    "supra_scanconv_destroy": (None, [C.c_void_p]),
+    "supra_scanconv_geometry": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "supra_scanconv_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "supra_scanconv_mask": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "supra_scanconv_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_void_p, C.c_int32,
+                                     C.c_void_p]),
     "supra_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.POINTER(C.c_void_p)]),
     "supra_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "supra_batch_latencies": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -78,7 +78,9 @@ class USRawData:
 
 @dataclass
 class USImage:
-    """Beamformed image: data [num_depths][num_scanlines] (scanline fastest)."""
+    """Beamformed image: data [num_depths][num_scanlines] (scanline fastest).
+    properties["geometry"], when present, is the RxGeometry whose scanlines
+    the image holds (USImageProperties::getScanlineInfo)."""
     data: "torch.Tensor"
     size: tuple                               # (num_scanlines, num_depths) as in USImage
     properties: dict = field(default_factory=dict)
@@ -130,7 +132,7 @@ class RxBeamformer:
             float(f_number), float(speed_of_sound_mm_per_s), int(window_type), float(window_parameter),
             int(bool(interpolate_transmits)), int(additional_offset), C.c_void_p(out.data_ptr()),
             _dtype_code(out_dtype), C.c_void_p(_stream_handle(stream))))
-        return USImage(out, (g.num_rx, g.num_depths), {"imageState": "RF"})
+        return USImage(out, (g.num_rx, g.num_depths), {"imageState": "RF", "geometry": g})
 
 
 class _Param:
@@ -296,7 +298,7 @@ def mv_das_perform_rx_beamforming(engine: "RxBeamformer", raw: USRawData, f_numb
         float(speed_of_sound_mm_per_s), int(window_type), float(window_parameter), int(sub_array_size),
         int(temporal_smoothing), float(sub_array_scaling_power), int(bool(compute_means)), float(loading_factor),
         C.c_void_p(out.data_ptr()), _dtype_code(out_dtype), C.c_void_p(_stream_handle(stream))))
-    return USImage(out, (g.num_rx, g.num_depths), {"imageState": "RF"})
+    return USImage(out, (g.num_rx, g.num_depths), {"imageState": "RF", "geometry": g})
 
 
 class RawDelayNode(BeamformingNode):

@@ -25,7 +25,7 @@ ARCH = os.environ.get("SUPRA_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["geometry.cpp", "bf_host.cpp", "status.cpp", "mock_io.cpp"]
 HIP_SOURCES = ["bf_context.hip", "das_kernels.hip", "das_adjoint.hip", "mv_kernels.hip", "synth.hip", "batch.hip",
-               "envelope.hip", "das_variants.hip"]
+               "envelope.hip", "das_variants.hip", "scan_converter.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_FLAGS = [

@@ -77,3 +77,18 @@ def test_library_window_matches_oracle():
     from supra_amd.beamformer import window_lut
     for wt, p in [(0, 0.0), (1, 0.0), (1, 0.3), (2, 0.0), (3, 0.5)]:
         assert np.array_equal(window_lut(wt, p), O.window_lut(wt, p, 32))
+
+
+def test_scanconv_rejects_bad_arguments(native):
+    from supra_amd.geometry import SCANLINE_DTYPE
+    s = np.zeros(4, SCANLINE_DTYPE)
+    sc = C.c_void_p()
+    for layout, depth, ns, flags in [((1, 1), 10.0, 100, 0), ((1, 4), 10.0, 100, 0), ((4, 1), 0.0, 100, 0),
+                                     ((4, 1), 10.0, 1, 0), ((4, 1), 10.0, 100, 2), ((0, 1), 10.0, 100, 0)]:
+        lay = (C.c_uint32 * 2)(*layout)
+        assert native.supra_scanconv_create(s.ctypes.data, lay, depth, ns, 0.0, flags, C.byref(sc)) == -1
+        assert not sc.value
+        assert b"scanconv" in native.supra_last_error()
+    assert native.supra_scanconv_run(None, None, 2, 1, None, 2, None) == -1
+    assert native.supra_scanconv_tables(None, None, None, None) == -1
+    assert native.supra_scanconv_geometry(None, None, None, None, None) == -1
