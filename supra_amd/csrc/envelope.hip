@@ -97,7 +97,9 @@ __global__ __launch_bounds__(64) void envelope_kernel(const EnvArgs a) {
         for (uint32_t j = 0; j < a.used_taps; j++) acc += win[(i + j) * 64u + lane] * a.h[j];
         const float v = win[(i + a.half) * 64u + lane];
         float env = sqrtf(v * v + acc * acc);
-        if (LOG) env = log_compress(env, a.in_scale, a.scale_den);
+        // the fused path sees the envelope as the float image the two-node
+        // chain passes on: clampCast<float> first (NaN -> -FLT_MAX, inf -> FLT_MAX)
+        if (LOG) env = log_compress(clamp_to<float>(env), a.in_scale, a.scale_den);
         static_cast<OutT*>(a.out)[static_cast<uint64_t>(t) * a.n_rx + s] = clamp_to<OutT>(env);
     }
 }
