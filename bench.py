@@ -248,6 +248,67 @@ def bench_diff(args, torch, frames: int = 16):
     return res
 
 
+def bench_bmode(args, torch, frames: int = 32):
+    """After the beamformer, per linear_2d frame (2080 x 128 float image):
+    envelope + log compression to uint8 in one launch per frame
+    (HilbertFirEnvelope 65 taps -> LogCompressor), then scan conversion of
+    all frames in one launch onto the default grid (pixel pitch = sample
+    distance, 1982 x 2080 pixels), uint8 -> uint8.  Algorithmic bytes: image
+    read + B-mode write for the first stage; for the second, the tables once
+    per launch (12 B/pixel) plus per frame the B-mode image and the output."""
+    from supra_amd import workloads as W
+    from supra_amd.beamformer import RxBeamformer, USRawData
+    from supra_amd.postprocess import envelope_log_compress
+    from supra_amd.scanconv import ScanConverter
+
+    w = W.linear_2d()
+    g = w.geometry()
+    rf = torch.empty((frames,) + w.rf_shape, dtype=torch.int16, device="cuda")
+    for f in range(frames):
+        W.linear_2d(scatter_seed=100 + f, noise_seed=200 + f).rf_device(
+            rf[f].data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
+    bf = RxBeamformer(g)
+    kw = w.das_kwargs()
+    img = bf.perform_rx_beamforming(0, USRawData(rf, w.fs_hz, g), kw["f_number"], kw["speed_of_sound_mm_per_s"],
+                                    kw["window_type"], kw["window_parameter"], kw["interpolate_transmits"],
+                                    kw["additional_offset"]).data
+    bmode = torch.empty((frames, g.num_depths, g.num_rx), dtype=torch.uint8, device="cuda")
+
+    def env_step():
+        for f in range(frames):
+            envelope_log_compress(img[f], 65, 80.0, 1.0, 20000.0, out_dtype=torch.uint8, out=bmode[f])
+
+    log(f"[bench] timing envelope + log compression, {frames} frames")
+    _, env_ms = time_launches(env_step, max(5, args.steps), 2, torch)
+    sc = ScanConverter()
+    t0 = time.perf_counter()
+    sc.update_internals(g)
+    tables_s = time.perf_counter() - t0
+    out = torch.empty((frames,) + sc._shape(), dtype=torch.uint8, device="cuda")
+
+    def sc_step():
+        sc.convert(bmode, torch.uint8, out=out)
+
+    log(f"[bench] timing scan conversion of {frames} frames onto {sc.image_size[:2]} pixels")
+    _, sc_ms = time_launches(sc_step, max(5, args.steps), 2, torch)
+    px = sc.num_pixels
+    img_px = g.num_depths * g.num_rx
+    env_bytes = frames * img_px * (4 + 1)
+    sc_bytes = px * 12 + frames * (img_px + px)
+    bf.close()
+    sc.close()
+    return {
+        "workload": f"linear_2d images 2080x128 float, {frames} frames; scan conversion to "
+                    f"{sc.image_size[0]}x{sc.image_size[1]} pixels, uint8",
+        "envelope_log_us_per_frame": env_ms * 1e3 / frames,
+        "envelope_log_hbm_frac": env_bytes / (env_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "scanconv_us_per_frame": sc_ms * 1e3 / frames,
+        "scanconv_achieved_GBs": sc_bytes / (sc_ms * 1e-3) / 1e9,
+        "scanconv_hbm_frac": sc_bytes / (sc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "scanconv_tables_s": tables_s,
+    }
+
+
 def diff_errors(dd, rf, c, target, tmask, g, w, torch):
     """The step's dL/drf (normwise) and dL/dc (max relative over frames)
     against the FP64 host reverse mode, on all pixels and with the pixels
@@ -510,6 +571,7 @@ def main() -> int:
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-mv", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-bmode", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) so that "
                          "FETCH_SIZE/WRITE_SIZE counters of a profile can be scaled")
@@ -550,6 +612,8 @@ def main() -> int:
         extra["mv_2d"] = bench_mv(args, torch)
     if not args.no_stream:
         extra["uneven_stream"] = bench_stream(args, torch)
+    if not args.no_bmode:
+        extra["bmode_2d"] = bench_bmode(args, torch)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_3d(res, args.cpu_budget)
