@@ -655,6 +655,8 @@ struct MvCall {
     uint32_t n_rx, n_ch, n_d, L, T;
     double p, alpha;
     int means, out_dtype;
+    uint32_t pcg_iters = 0;   // 0: LU solve; else the PCG solve (solve_pcg) with at most this many iterations
+    double pcg_threshold = 0;
 };
 
 inline double x_at(const MvCall& m, uint64_t idx) {
@@ -687,6 +689,52 @@ void solve_lu(double* A, double* b, int n) {
         for (int k = r + 1; k < n; k++) acc -= A[r * n + k] * b[k];
         b[r] = acc / A[r * n + r];
     }
+}
+
+// RxBeamformerMVpcg's solve (solveBlockwisePCGJacobi, RxBeamformerMVpcg.cu:
+// 450-546, reinitialisation frequency 1 as passed at :773-783) in FP64 for
+// A w = 1 (n x n, row-major): x0 = 1, r = 1 - A x recomputed every
+// iteration, Jacobi preconditioner diag(A), at most max_iter iterations,
+// stopping once r.M^-1 r <= 1e-30 or |r| <= threshold.  Returns the
+// iterations taken.
+// Pinned by: tests/test_oracle_mv.py — with n iterations and no threshold it
+// equals the LU solve (conjugate gradients end in n steps), the A-norm of the
+// error never grows from one iteration count to the next, and one iteration
+// solves a plane-wave system (1 is an eigenvector) exactly.
+uint32_t solve_pcg(const double* A, double* w, int n, uint32_t max_iter, double threshold) {
+    std::vector<double> x(n, 1.0), r(n), d(n), q(n), sv(n);
+    auto matvec = [&](const std::vector<double>& v, std::vector<double>& out) {
+        for (int i = 0; i < n; i++) {
+            double acc = 0;
+            for (int k = 0; k < n; k++) acc += A[i * n + k] * v[k];
+            out[i] = acc;
+        }
+    };
+    auto dotp = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double acc = 0;
+        for (int i = 0; i < n; i++) acc += a[i] * b[i];
+        return acc;
+    };
+    matvec(x, q);
+    for (int i = 0; i < n; i++) r[i] = 1.0 - q[i];
+    for (int i = 0; i < n; i++) d[i] = r[i] / A[i * n + i];
+    double dnew = dotp(r, d), rnorm = 2 * threshold;
+    uint32_t it = 0;
+    for (; it < max_iter && dnew > 1e-30 && rnorm > threshold; it++) {
+        matvec(d, q);
+        const double alpha = dnew / dotp(d, q);
+        for (int i = 0; i < n; i++) x[i] += alpha * d[i];
+        matvec(x, q);
+        for (int i = 0; i < n; i++) r[i] = 1.0 - q[i];
+        rnorm = std::sqrt(dotp(r, r));
+        for (int i = 0; i < n; i++) sv[i] = r[i] / A[i * n + i];
+        const double dold = dnew;
+        dnew = dotp(r, sv);
+        const double beta = dnew / dold;
+        for (int i = 0; i < n; i++) d[i] = sv[i] + beta * d[i];
+    }
+    for (int i = 0; i < n; i++) w[i] = x[i];
+    return it;
 }
 
 void mv_scanline(const MvCall& m, uint32_t s, double* out, uint32_t dbg_sample, double* dbg_A, double* dbg_w,
@@ -741,8 +789,13 @@ void mv_scanline(const MvCall& m, uint32_t s, double* out, uint32_t dbg_sample, 
         const double delta = m.alpha * (trace / n);
         for (uint32_t i = 0; i < n; i++) A[i * n + i] += delta;
         if (t == dbg_sample && dbg_A) std::copy(A.begin(), A.begin() + n * n, dbg_A);
-        for (uint32_t i = 0; i < n; i++) w[i] = 1.0;
-        solve_lu(A.data(), w.data(), static_cast<int>(n));
+        uint32_t iters = 0;
+        if (m.pcg_iters) {
+            iters = solve_pcg(A.data(), w.data(), static_cast<int>(n), m.pcg_iters, m.pcg_threshold);
+        } else {
+            for (uint32_t i = 0; i < n; i++) w[i] = 1.0;
+            solve_lu(A.data(), w.data(), static_cast<int>(n));
+        }
         double wsum = 0;
         for (uint32_t i = 0; i < n; i++) wsum += w[i];
         const double scaling = 1.0 / (wsum * std::pow(static_cast<double>(ms.Kact), m.p));
@@ -760,6 +813,7 @@ void mv_scanline(const MvCall& m, uint32_t s, double* out, uint32_t dbg_sample, 
                 dbg_meta[1] = static_cast<int32_t>(ms.last);
                 dbg_meta[2] = static_cast<int32_t>(ms.Lt);
                 dbg_meta[3] = static_cast<int32_t>(ms.Kact);
+                dbg_meta[4] = static_cast<int32_t>(iters);
             }
         }
         out[static_cast<uint64_t>(t) * m.n_rx + s] = clamp_out(y * n_ch, m.out_dtype);
@@ -1143,6 +1197,29 @@ int oracle_mv(const void* x, int dtype, uint32_t n_rx, uint32_t n_ch, uint32_t n
     if (L == 0 || L > n_ch) return -1;
     MvCall m{x, dtype, n_rx, n_ch, n_d, L, temporal_smoothing, scaling_power, loading_factor, compute_means,
              out_dtype};
+    parallel_for(static_cast<int64_t>(n_rx), nthreads, [&](int64_t s) {
+        const bool dbg = static_cast<uint32_t>(s) == debug_scanline;
+        mv_scanline(m, static_cast<uint32_t>(s), out, dbg ? debug_sample : 0xffffffffu, dbg ? debug_A : nullptr,
+                    dbg ? debug_w : nullptr, dbg ? debug_meta : nullptr);
+    });
+    return 0;
+}
+
+// The same with RxBeamformerMVpcg's solve (loading trace/L_t; max_iterations
+// 0 means the sub-array size, RxBeamformerMVpcg.cu:649).  debug_meta gets a
+// fifth entry: the PCG iterations of the debug sample.
+int oracle_mvpcg(const void* x, int dtype, uint32_t n_rx, uint32_t n_ch, uint32_t n_d, uint32_t sub_array_size,
+                 uint32_t temporal_smoothing, uint32_t max_iterations, double threshold, double scaling_power,
+                 int compute_means, int out_dtype, double* out, int nthreads, uint32_t debug_scanline,
+                 uint32_t debug_sample, double* debug_A, double* debug_w, int32_t* debug_meta) {
+    if (!x || !out || n_rx == 0 || n_ch == 0 || n_d == 0) return -1;
+    uint32_t L = sub_array_size == 0 ? n_ch / 2 : sub_array_size;
+    if (L == 0 || L > n_ch) return -1;
+    MvCall m{x, dtype, n_rx, n_ch, n_d, L, temporal_smoothing, scaling_power, 1.0, compute_means, out_dtype};
+    m.pcg_iters = max_iterations ? max_iterations : L;
+    // the reference passes the threshold to its kernel as a float (:782): a
+    // threshold below the float range is 0, and then no iteration runs
+    m.pcg_threshold = static_cast<double>(static_cast<float>(threshold));
     parallel_for(static_cast<int64_t>(n_rx), nthreads, [&](int64_t s) {
         const bool dbg = static_cast<uint32_t>(s) == debug_scanline;
         mv_scanline(m, static_cast<uint32_t>(s), out, dbg ? debug_sample : 0xffffffffu, dbg ? debug_A : nullptr,

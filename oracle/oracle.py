@@ -47,6 +47,10 @@ def lib() -> C.CDLL:
         h.oracle_mv.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.c_double, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int, C.c_uint32,
                                 C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        h.oracle_mvpcg.restype = C.c_int
+        h.oracle_mvpcg.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
         h.oracle_envelope.restype = C.c_int
         h.oracle_envelope.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p,
                                       C.c_int]
@@ -183,7 +187,7 @@ def mv(x: np.ndarray, sub_array_size: int = 0, temporal_smoothing: int = 3, scal
     out = np.zeros((n_d, n_rx))
     dA = np.zeros(L * L)
     dw = np.zeros(L)
-    dm = np.zeros(4, dtype=np.int32)
+    dm = np.zeros(5, dtype=np.int32)
     ds, dt = debug if debug else (0xFFFFFFFF, 0xFFFFFFFF)
     code = lib().oracle_mv(x.ctypes.data, 1 if x.dtype == np.int16 else 2, n_rx, n_ch, n_d, sub_array_size,
                            temporal_smoothing, scaling_power, int(compute_means), loading_factor,
@@ -191,6 +195,33 @@ def mv(x: np.ndarray, sub_array_size: int = 0, temporal_smoothing: int = 3, scal
                            dA.ctypes.data, dw.ctypes.data, dm.ctypes.data)
     if code != 0:
         raise ValueError(f"oracle_mv rejected its arguments ({code})")
+    if debug:
+        n = int(dm[2])
+        return out, dA[:n * n].reshape(n, n), dw[:n], tuple(int(v) for v in dm[:4])
+    return out
+
+
+def mvpcg(x: np.ndarray, sub_array_size: int = 0, temporal_smoothing: int = 3, max_iterations: int = 0,
+          threshold: float = 1e-4, scaling_power: float = 1.5, compute_means: bool = False, out_int16: bool = False,
+          threads: int | None = None, debug: tuple | None = None):
+    """Oracle minimum variance with RxBeamformerMVpcg's PCG solve (FP64).
+    debug=(scanline, sample) also returns (A, w, (first, last, L_t, K_act,
+    iterations)) of that sample."""
+    x = np.ascontiguousarray(x)
+    assert x.ndim == 3 and x.dtype in (np.int16, np.float32)
+    n_rx, n_ch, n_d = x.shape
+    L = sub_array_size or n_ch // 2
+    out = np.zeros((n_d, n_rx))
+    dA = np.zeros(L * L)
+    dw = np.zeros(L)
+    dm = np.zeros(5, dtype=np.int32)
+    ds, dt = debug if debug else (0xFFFFFFFF, 0xFFFFFFFF)
+    code = lib().oracle_mvpcg(x.ctypes.data, 1 if x.dtype == np.int16 else 2, n_rx, n_ch, n_d, sub_array_size,
+                              temporal_smoothing, max_iterations, threshold, scaling_power, int(compute_means),
+                              1 if out_int16 else 2, out.ctypes.data, threads or default_threads(), ds, dt,
+                              dA.ctypes.data, dw.ctypes.data, dm.ctypes.data)
+    if code != 0:
+        raise ValueError(f"oracle_mvpcg rejected its arguments ({code})")
     if debug:
         n = int(dm[2])
         return out, dA[:n * n].reshape(n, n), dw[:n], tuple(int(v) for v in dm)

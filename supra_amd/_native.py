@@ -154,6 +154,9 @@ _SIGNATURES = {
                                      C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
     "supra_mv_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                C.c_double, C.c_int32, C.c_double, C.c_void_p, C.c_int32, C.c_void_p]),
+    "supra_mvpcg_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_int32, C.c_void_p, C.c_int32,
+                                  C.c_void_p]),
     "supra_mv_das_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_double, C.c_double, C.c_double, C.c_int32, C.c_float, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_int32, C.c_double, C.c_void_p, C.c_int32,
@@ -164,11 +167,13 @@ _SIGNATURES = {
     "supra_mock_write_ascii": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(RxGeometryC)]),
     "supra_mock_count_frames": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     "supra_mock_read_frames": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
-    "supra_envelope_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+    "supra_envelope_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_void_p,
                                      C.c_int32, C.c_void_p]),
     "supra_logcompress_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_double, C.c_double, C.c_double,
                                         C.c_void_p, C.c_int32, C.c_void_p]),
     "supra_envelope_logcompress_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_uint32,
                                                  C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int32,
                                                  C.c_void_p]),
     "supra_scanconv_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_double, C.c_uint32, C.c_double,

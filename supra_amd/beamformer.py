@@ -161,7 +161,7 @@ class BeamformingNode:
         "additionalOffset": _Param(0, -1000, 1000),
         "outputType": _Param("float", choices=set(OUTPUT_TYPES)),
     }
-    # north-star spelling of the beamformer key, accepted as an alias
+    # the spelling rxBeamformerType is accepted as an alias of the beamformer key
     ALIASES = {"rxBeamformerType": "beamformerType"}
 
     def __init__(self, node_id: str = "BeamformingNode", **params):
@@ -276,6 +276,30 @@ def mv_perform_rx_beamforming(raw: USRawData, sub_array_size: int = 0, temporal_
     return USImage(out, (n_rx, n_s), {"imageState": "RF"})
 
 
+def mvpcg_perform_rx_beamforming(raw: USRawData, sub_array_size: int = 0, temporal_smoothing: int = 3,
+                                 max_iterations_override: int = 0, convergence_threshold: float = 1e-4,
+                                 sub_array_scaling_power: float = 1.5, compute_means: bool = False,
+                                 out_dtype=None, out=None, stream=None) -> USImage:
+    """RxBeamformerMVpcg::performRxBeamforming (RxBeamformerMVpcg.cu:628-800):
+    minimum variance with the weights from Jacobi-preconditioned conjugate
+    gradients, on pre-delayed data [scanline][channel][sample]."""
+    data = raw.data
+    if data.dim() != 3 or not data.is_cuda or not data.is_contiguous():
+        raise ValueError("raw.data must be a contiguous device tensor [scanline][channel][sample]")
+    n_rx, n_ch, n_s = (int(v) for v in data.shape)
+    out_dtype = out_dtype or torch.float32
+    if out is None:
+        out = torch.empty((n_s, n_rx), dtype=out_dtype, device=data.device)
+    elif tuple(out.shape) != (n_s, n_rx) or out.dtype != out_dtype or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {(n_s, n_rx)}")
+    N.check(N.lib().supra_mvpcg_run(C.c_void_p(data.data_ptr()), _dtype_code(data.dtype), n_rx, n_ch, n_s,
+                                    int(sub_array_size), int(temporal_smoothing), int(max_iterations_override),
+                                    float(convergence_threshold), float(sub_array_scaling_power),
+                                    int(bool(compute_means)), C.c_void_p(out.data_ptr()), _dtype_code(out_dtype),
+                                    C.c_void_p(_stream_handle(stream))))
+    return USImage(out, (n_rx, n_s), {"imageState": "RF"})
+
+
 def mv_das_perform_rx_beamforming(engine: "RxBeamformer", raw: USRawData, f_number: float,
                                   speed_of_sound_mm_per_s: float, window_type: int, window_parameter: float,
                                   sub_array_size: int = 0, temporal_smoothing: int = 3,
@@ -344,6 +368,30 @@ class BeamformingMVNode(BeamformingNode):
         return mv_perform_rx_beamforming(raw, self.config["subArraySize"], self.config["temporalSmoothing"],
                                          self.config["subArrayScalingPower"], self.config["computeMeans"],
                                          self.config["loadingFactor"], out_dtype=out_dtype, stream=stream)
+
+
+class BeamformingMVpcgNode(BeamformingNode):
+    """BeamformingMVpcgNode (BeamformingMVpcgNode.cpp:38-193): parameters
+    :55-61 (the convergence threshold is 10^convergenceThresholdExponent)."""
+
+    PARAMETERS = {
+        "subArraySize": _Param(0, 0, 64),
+        "temporalSmoothing": _Param(3, 0, 10),
+        "outputType": _Param("float", choices=set(OUTPUT_TYPES)),
+        "maxIterationsOverride": _Param(0, 0, 10000),
+        "convergenceThresholdExponent": _Param(-4.0, -100.0, 0.0),
+        "subArrayScalingPower": _Param(1.5, 0.5, 3.0),
+        "computeMeans": _Param(False, choices={False, True}),
+    }
+    ALIASES = {}
+
+    def process(self, raw: USRawData, stream=None) -> USImage:
+        """checkTypeAndBeamform (BeamformingMVpcgNode.cpp:147-190); input must be pre-delayed."""
+        out_dtype = torch.float32 if self.config["outputType"] == "float" else torch.int16
+        return mvpcg_perform_rx_beamforming(
+            raw, self.config["subArraySize"], self.config["temporalSmoothing"],
+            self.config["maxIterationsOverride"], 10.0 ** self.config["convergenceThresholdExponent"],
+            self.config["subArrayScalingPower"], self.config["computeMeans"], out_dtype=out_dtype, stream=stream)
 
 
 def window_lut(window_type: int, window_parameter: float) -> np.ndarray:

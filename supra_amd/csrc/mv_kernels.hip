@@ -215,6 +215,55 @@ __device__ inline float chol_solve(const float (&a)[LMAX], float ginv, float b, 
     return v;
 }
 
+// Jacobi-preconditioned conjugate gradients for R~ w = 1 on the sub-wave's
+// pixel, as solveBlockwisePCGJacobi (RxBeamformerMVpcg.cu:450-546) with the
+// reinitialisation frequency of 1 that performRxBeamforming passes (:773-783):
+// x0 = 1, the residual recomputed as 1 - R~ x every iteration, at most
+// max_iter iterations, stopping once r.M^-1 r <= 1e-30 or |r| <= threshold
+// (M = diag R~).  Lane i holds row i of R~ in A[] (entries k >= n unused)
+// and its diagonal in mdiag; lanes i >= n carry no data.  Both sub-waves of
+// a wave run the same number of steps; a finished one keeps its x.
+template <int LMAX, int PPW>
+__device__ inline float pcg_solve(const float (&A)[LMAX], float mdiag, int n, int li, int half, uint32_t max_iter,
+                                  float threshold) {
+    const bool mine = li < n;
+    auto matvec = [&](float v) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) {
+            const float vk = sub_bcast_swz<PPW>(v, k, half);
+            if (k < n) acc = fmaf(A[k], vk, acc);
+        }
+        return acc;
+    };
+    float x = mine ? 1.0f : 0.0f;
+    const float ax0 = matvec(x);
+    float r = mine ? 1.0f - ax0 : 0.0f;
+    float d = mine ? r / mdiag : 0.0f;
+    float dnew = sub_sum<PPW>(mine ? r * d : 0.0f);
+    float rnorm = 2.0f * threshold;
+    bool active = max_iter > 0 && dnew > 1e-30f && rnorm > threshold;
+    for (uint32_t it = 0; __any(active);) {
+        const float q = matvec(d);
+        const float alpha = dnew / sub_sum<PPW>(mine ? d * q : 0.0f);
+        if (active) x = fmaf(alpha, d, x);
+        const float ax = matvec(x);
+        if (active) r = mine ? 1.0f - ax : 0.0f;
+        const float rr = sub_sum<PPW>(mine ? r * r : 0.0f);
+        const float sv = mine ? r / mdiag : 0.0f;
+        const float dn = sub_sum<PPW>(mine ? r * sv : 0.0f);
+        if (active) {
+            rnorm = sqrtf(rr);
+            const float beta = dn / dnew;
+            dnew = dn;
+            d = fmaf(beta, d, sv);
+        }
+        it++;
+        active = active && it < max_iter && dnew > 1e-30f && rnorm > threshold;
+    }
+    return x;
+}
+
 // Fused mode: the pre-delayed samples of scanline s, depths [h_lo, h_lo+H),
 // computed from the RF exactly as the pre-delay kernel writes them
 // (rawdelay2d_kernel in das_kernels.hip; RawDelay.cu:221-312, 404-468):
@@ -536,87 +585,94 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
         float orig[LMAX];
 #pragma unroll
         for (int k = 0; k < LMAX; k++) orig[k] = row[k] + ((k == li) ? delta : 0.0f);
-        // Cholesky, row `li` in registers, FP64 pivots and diagonal.  Row
-        // entries k >= li are never read by the factorisation, so the loading
-        // enters only through the diagonal dd.
-        double dd = static_cast<double>(diag + delta);
-        // steps beyond the larger system of the wave are skipped (uniform branch)
-        const int nmax = (PPW == 1) ? n : max(__builtin_amdgcn_readlane(n, 0), __builtin_amdgcn_readlane(n, 32));
-#pragma unroll
-        for (int j = 0; j < LMAX; j++) {
-            if (j < nmax) {   // a guard, not a break: the loop must unroll fully (row[j] in registers)
-                // 1/sqrt of the lane's own running diagonal in FP64 (hardware
-                // estimate and one Newton step); at step j lane j's value is
-                // the pivot's, broadcast as the float the rows are scaled by
-                double r = __builtin_amdgcn_rsq(dd);
-                r = r * (1.5 - (0.5 * dd) * (r * r));
-                const float invg = sub_bcast<PPW>(static_cast<float>(r), j, half);
-                if (j < n) {
-                    if (li > j && li < n) row[j] *= invg;
-                    if (li > j && li < n) lmat[lrow + j] = row[j];
+        // w = R~^-1 1: Cholesky (RxBeamformerMV) or, with pcg_iters > 0,
+        // Jacobi-preconditioned conjugate gradients (RxBeamformerMVpcg)
+        float w;
+        if (a.pcg_iters == 0) {
+            // Cholesky, row `li` in registers, FP64 pivots and diagonal.  Row
+            // entries k >= li are never read by the factorisation, so the loading
+            // enters only through the diagonal dd.
+            double dd = static_cast<double>(diag + delta);
+            // steps beyond the larger system of the wave are skipped (uniform branch)
+            const int nmax = (PPW == 1) ? n : max(__builtin_amdgcn_readlane(n, 0), __builtin_amdgcn_readlane(n, 32));
+    #pragma unroll
+            for (int j = 0; j < LMAX; j++) {
+                if (j < nmax) {   // a guard, not a break: the loop must unroll fully (row[j] in registers)
+                    // 1/sqrt of the lane's own running diagonal in FP64 (hardware
+                    // estimate and one Newton step); at step j lane j's value is
+                    // the pivot's, broadcast as the float the rows are scaled by
+                    double r = __builtin_amdgcn_rsq(dd);
+                    r = r * (1.5 - (0.5 * dd) * (r * r));
+                    const float invg = sub_bcast<PPW>(static_cast<float>(r), j, half);
+                    if (j < n) {
+                        if (li > j && li < n) row[j] *= invg;
+                        if (li > j && li < n) lmat[lrow + j] = row[j];
+                    }
+                    // Branch-free update of the lanes below the pivot (others take a
+                    // zero multiplier).  Entries k >= li of row li are its diagonal
+                    // (tracked in dd) and upper part, which are never read again, so
+                    // they are updated unmasked.  G[k][j] is lane k's row[j]: with
+                    // one system per wave it is broadcast straight from that lane
+                    // (no LDS round trip per step); two systems per wave read it
+                    // back from the factor in LDS.
+                    const float nlij = (j < n && li > j && li < n) ? -row[j] : 0.0f;
+                    if constexpr (PPW == 1) {
+                        float gcol[LMAX];
+    #pragma unroll
+                        for (int k = j + 1; k < LMAX; k++) gcol[k] = bcast(row[j], k);
+    #pragma unroll
+                        for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, gcol[k], row[k]);
+                    } else {
+                        wave_sync();
+    #pragma unroll
+                        for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, lmat[(k * (k - 1)) / 2 + j], row[k]);
+                    }
+                    dd -= static_cast<double>(nlij) * static_cast<double>(nlij);
                 }
-                // Branch-free update of the lanes below the pivot (others take a
-                // zero multiplier).  Entries k >= li of row li are its diagonal
-                // (tracked in dd) and upper part, which are never read again, so
-                // they are updated unmasked.  G[k][j] is lane k's row[j]: with
-                // one system per wave it is broadcast straight from that lane
-                // (no LDS round trip per step); two systems per wave read it
-                // back from the factor in LDS.
-                const float nlij = (j < n && li > j && li < n) ? -row[j] : 0.0f;
-                if constexpr (PPW == 1) {
-                    float gcol[LMAX];
-#pragma unroll
-                    for (int k = j + 1; k < LMAX; k++) gcol[k] = bcast(row[j], k);
-#pragma unroll
-                    for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, gcol[k], row[k]);
-                } else {
-                    wave_sync();
-#pragma unroll
-                    for (int k = j + 1; k < LMAX; k++) row[k] = fmaf(nlij, lmat[(k * (k - 1)) / 2 + j], row[k]);
+            }
+            // Row li's diagonal stops changing after step li - 1, so dd is now
+            // lane li's pivot: its 1/G[li][li] (the value broadcast at step li)
+            // and whether every pivot of the system was positive.
+            double rg = __builtin_amdgcn_rsq(dd);
+            rg = rg * (1.5 - (0.5 * dd) * (rg * rg));
+            const float ginv = static_cast<float>(rg);
+            const uint64_t nonpos = __ballot(li < n && !(dd > 0.0));
+            const bool ok = (PPW == 1) ? nonpos == 0 : ((nonpos >> (32 * half)) & 0xffffffffull) == 0;
+            wave_sync();
+            w = chol_solve<LMAX, PPW>(row, ginv, 1.0f, n, li, half, lmat);
+            // FP64 residual r = R~ w - 1 and up to two refinement steps, only for
+            // systems that may be ill-conditioned.  R~ is a sum of outer products
+            // (positive semi-definite), so with the loading delta = alpha trace/n
+            // its condition number is at most (trace + delta) / delta = 1 + n/alpha;
+            // the ratio of the largest to the smallest squared pivot is a lower
+            // bound on it.  Below kKappaSafe (bound) or kPivotRatioSafe (pivots)
+            // the FP32 factorisation already solves to about kappa * 6e-8.
+            float dmax = (li < n) ? static_cast<float>(dd) : 0.0f;
+            float dmin = (li < n) ? static_cast<float>(dd) : FLT_MAX;
+            dmax = sub_max<PPW>(dmax);
+            dmin = sub_min<PPW>(dmin);
+            const bool loaded_ok = a.alpha > 0.0f && 1.0f + static_cast<float>(n) / a.alpha <= kKappaSafe;
+            const bool suspect = ok && !loaded_ok && !(dmax <= kPivotRatioSafe * dmin);
+            for (int it = 0; it < 3 && __any(suspect); it++) {
+                double r = 0.0;
+    #pragma unroll
+                for (int k = 0; k < LMAX; k++) {
+                    const double wk = static_cast<double>(sub_bcast_swz<PPW>(w, k, half));
+                    if (k < n) r += static_cast<double>(orig[k]) * wk;
                 }
-                dd -= static_cast<double>(nlij) * static_cast<double>(nlij);
+                r = (li < n) ? r - 1.0 : 0.0;
+                const double rn = sub_sum<PPW>(r * r);
+                // refine only where single precision left a relative residual
+                // above 1e-3 (ill-conditioned systems); both halves of a wave
+                // take the same number of steps
+                const bool more = (rn > 1e-6 * n) && suspect;
+                const bool any_more = __any(more);
+                if (!any_more || it == 2) break;
+                const float dw = chol_solve<LMAX, PPW>(row, ginv, static_cast<float>(r), n, li, half, lmat);
+                if (more) w -= dw;
             }
-        }
-        // Row li's diagonal stops changing after step li - 1, so dd is now
-        // lane li's pivot: its 1/G[li][li] (the value broadcast at step li)
-        // and whether every pivot of the system was positive.
-        double rg = __builtin_amdgcn_rsq(dd);
-        rg = rg * (1.5 - (0.5 * dd) * (rg * rg));
-        const float ginv = static_cast<float>(rg);
-        const uint64_t nonpos = __ballot(li < n && !(dd > 0.0));
-        const bool ok = (PPW == 1) ? nonpos == 0 : ((nonpos >> (32 * half)) & 0xffffffffull) == 0;
-        wave_sync();
-        float w = chol_solve<LMAX, PPW>(row, ginv, 1.0f, n, li, half, lmat);
-        // FP64 residual r = R~ w - 1 and up to two refinement steps, only for
-        // systems that may be ill-conditioned.  R~ is a sum of outer products
-        // (positive semi-definite), so with the loading delta = alpha trace/n
-        // its condition number is at most (trace + delta) / delta = 1 + n/alpha;
-        // the ratio of the largest to the smallest squared pivot is a lower
-        // bound on it.  Below kKappaSafe (bound) or kPivotRatioSafe (pivots)
-        // the FP32 factorisation already solves to about kappa * 6e-8.
-        float dmax = (li < n) ? static_cast<float>(dd) : 0.0f;
-        float dmin = (li < n) ? static_cast<float>(dd) : FLT_MAX;
-        dmax = sub_max<PPW>(dmax);
-        dmin = sub_min<PPW>(dmin);
-        const bool loaded_ok = a.alpha > 0.0f && 1.0f + static_cast<float>(n) / a.alpha <= kKappaSafe;
-        const bool suspect = ok && !loaded_ok && !(dmax <= kPivotRatioSafe * dmin);
-        for (int it = 0; it < 3 && __any(suspect); it++) {
-            double r = 0.0;
-#pragma unroll
-            for (int k = 0; k < LMAX; k++) {
-                const double wk = static_cast<double>(sub_bcast_swz<PPW>(w, k, half));
-                if (k < n) r += static_cast<double>(orig[k]) * wk;
-            }
-            r = (li < n) ? r - 1.0 : 0.0;
-            const double rn = sub_sum<PPW>(r * r);
-            // refine only where single precision left a relative residual
-            // above 1e-3 (ill-conditioned systems); both halves of a wave
-            // take the same number of steps
-            const bool more = (rn > 1e-6 * n) && suspect;
-            const bool any_more = __any(more);
-            if (!any_more || it == 2) break;
-            const float dw = chol_solve<LMAX, PPW>(row, ginv, static_cast<float>(r), n, li, half, lmat);
-            if (more) w -= dw;
+        } else {
+            w = pcg_solve<LMAX, PPW>(orig, diag + delta, n, li, half, a.pcg_iters, a.pcg_threshold);
         }
         // weights -> sample (applyWeights, RxBeamformerMV.cu:342-376)
         float xsum = 0.0f;
@@ -801,6 +857,8 @@ hipError_t launch_mv_batch(const MvBatchArgs& b, uint32_t lmax, int32_t rf_dtype
     return hipGetLastError();
 }
 
+size_t mv_batch_frame_bytes() { return sizeof(MvBatchFrame); }
+
 int launch_mv_fused(const DasArgs& g, int32_t rf_dtype, uint32_t sub_array_size, uint32_t temporal_smoothing,
                     double sub_array_scaling_power, int32_t compute_means, double loading_factor, void* out,
                     int32_t out_dtype, hipStream_t stream) {
@@ -838,5 +896,36 @@ extern "C" int supra_mv_run(const void* raw_delayed, int32_t in_dtype, uint32_t 
     if (st != SUPRA_OK) return st;
     hipError_t e = launch_io<false>(a, in_dtype, out_dtype, lds, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(SUPRA_ERR_HIP, "mv: launch: %s", hipGetErrorString(e));
+    return SUPRA_OK;
+}
+
+extern "C" int supra_mvpcg_run(const void* raw_delayed, int32_t in_dtype, uint32_t num_scanlines,
+                               uint32_t num_channels, uint32_t num_samples, uint32_t sub_array_size,
+                               uint32_t temporal_smoothing, uint32_t max_iterations_override,
+                               double convergence_threshold, double sub_array_scaling_power, int32_t compute_means,
+                               void* out, int32_t out_dtype, void* stream) {
+    clear_error();
+    if (!raw_delayed || !out) return fail(SUPRA_ERR_INVALID_ARG, "mvpcg: null argument");
+    if ((in_dtype != SUPRA_INT16 && in_dtype != SUPRA_FLOAT32) ||
+        (out_dtype != SUPRA_INT16 && out_dtype != SUPRA_FLOAT32))
+        return fail(SUPRA_ERR_UNSUPPORTED, "mvpcg: element types must be SUPRA_INT16 or SUPRA_FLOAT32");
+    if (!(convergence_threshold >= 0) || !std::isfinite(convergence_threshold))
+        return fail(SUPRA_ERR_INVALID_ARG, "mvpcg: convergence threshold %g", convergence_threshold);
+    if (max_iterations_override > 10000)
+        return fail(SUPRA_ERR_INVALID_ARG, "mvpcg: at most 10000 iterations (%u)", max_iterations_override);
+    MvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = raw_delayed;
+    a.out = out;
+    size_t lds = 0;
+    // loading trace / L_t as addDiagonalLoading (RxBeamformerMVpcg.cu:295-330)
+    int st = mv_prepare(a, num_scanlines, num_channels, num_samples, sub_array_size, temporal_smoothing,
+                        sub_array_scaling_power, compute_means, 1.0, 0, &lds);
+    if (st != SUPRA_OK) return st;
+    // maxIterations = the sub-array size unless overridden (:649)
+    a.pcg_iters = max_iterations_override ? max_iterations_override : a.L;
+    a.pcg_threshold = static_cast<float>(convergence_threshold);
+    hipError_t e = launch_io<false>(a, in_dtype, out_dtype, lds, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(SUPRA_ERR_HIP, "mvpcg: launch: %s", hipGetErrorString(e));
     return SUPRA_OK;
 }

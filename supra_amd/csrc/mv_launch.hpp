@@ -22,6 +22,8 @@ struct MvArgs {
     uint32_t off_fused;     // LDS byte offset of the pre-delay tables (fused mode)
     float p, alpha;
     int means;
+    uint32_t pcg_iters;     // 0: Cholesky solve; else at most this many PCG iterations
+    float pcg_threshold;    // PCG stops once |1 - R~ w| <= this
     DasArgs g;              // fused mode: geometry and RF of the pre-delay
 };
 
@@ -39,6 +41,11 @@ struct MvBatchArgs {
     uint32_t n_items;
     BatchSync sync;
 };
+
+// sizeof(MvBatchFrame) as the kernels' translation unit sees it: a batch
+// checks its own view against it before it uploads a frame table (objects
+// built from different revisions of the struct must not be linked together).
+size_t mv_batch_frame_bytes();
 
 // Plan (tile of output depths, waves, LDS offsets) for a frame; lds_budget 0
 // picks the single-launch default.  Returns a SUPRA_* status.

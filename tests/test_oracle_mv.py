@@ -167,3 +167,66 @@ def test_mv_on_predelayed_frame_is_finite():
     rd = O.rawdelay(g, rf, w.fs_hz, 1.0, 1.54e6, N.WINDOW_HANN)
     out = O.mv(rd[:8], 32, 5, 1.5, False, 0.01)
     assert np.isfinite(out).all() and np.abs(out).max() > 0
+
+
+def test_mvpcg_full_iterations_equal_the_direct_solve():
+    """Conjugate gradients end in n steps: with L_t iterations and no
+    threshold the PCG oracle is the LU oracle (FP64)."""
+    x, _, _ = random_apertures(seed=3)
+    for L, T in ((16, 2), (32, 0)):
+        a = O.mv(x, L, T, 1.5, False, 1.0)
+        b = O.mvpcg(x, L, T, max_iterations=L, threshold=1e-30)
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(a).max()
+
+
+def test_mvpcg_error_a_norm_never_grows():
+    """The A-norm of the error of (preconditioned) conjugate gradients does
+    not grow with the iteration count; w_k from k = 1..n iterations of one
+    sample, w* from numpy.linalg.solve on the oracle's own system."""
+    x, _, _ = random_apertures(seed=4)
+    for (s, t) in ((0, 20), (2, 41)):
+        _, A, w_star_o, meta = O.mvpcg(x, 16, 2, max_iterations=16, threshold=1e-30, debug=(s, t))
+        w_star = np.linalg.solve(A, np.ones(len(A)))
+        errs = []
+        for k in range(1, len(A) + 1):
+            _, _, wk, mk = O.mvpcg(x, 16, 2, max_iterations=k, threshold=1e-30, debug=(s, t))
+            assert mk[4] <= k
+            e = wk - w_star
+            errs.append(e @ A @ e)
+            if mk[4] < k:          # converged (r.M^-1 r <= 1e-30) before the budget
+                break
+        errs = np.array(errs)
+        assert np.all(np.diff(errs) <= 1e-9 * errs[0]), errs
+        assert errs[-1] <= 1e-16 * errs[0]
+
+
+@pytest.mark.parametrize("T,p", [(3, 1.5), (0, 1.0)])
+def test_mvpcg_plane_wave_one_iteration(T, p):
+    """All channels equal: 1 is an eigenvector of R~ + delta I and of its
+    diagonal, so one PCG step solves the system and the output is the MV
+    closed form N_ch s(t) K^(1-p)."""
+    x, s = plane_wave()
+    n_ch, L = 64, 32
+    K = n_ch - L + 1
+    out = O.mvpcg(x, L, T, max_iterations=1, threshold=1e-30, scaling_power=p)
+    assert np.allclose(out, (n_ch * s * K ** (1 - p)).T, rtol=1e-9)
+
+
+def test_mvpcg_threshold_below_float_range_runs_no_iteration():
+    """The reference hands its kernel (float)threshold: 10^-50 becomes 0 and
+    the loop (residual 2 * threshold > threshold) never starts, so w = 1 and
+    y = sum_i X_i / (L_t K^p), the infinite-loading closed form."""
+    x, f, l = random_apertures(seed=5)
+    n_rx, n_ch, n_d = x.shape
+    L, p = 32, 1.5
+    K = n_ch - L + 1
+    out = O.mvpcg(x, L, 2, threshold=1e-50, scaling_power=p)
+    _, _, _, meta = O.mvpcg(x, L, 2, threshold=1e-50, scaling_power=p, debug=(1, 7))
+    assert meta[4] == 0
+    for s in range(n_rx):
+        for t in range(n_d):
+            lt = min(L, l[s, t] - f[s, t] + 1)
+            kact = min(K, l[s, t] - f[s, t] - lt + 2)
+            X = sum(x[s, f[s, t] + k: f[s, t] + k + lt, t].astype(np.float64) for k in range(kact))
+            y = X.sum() / (lt * kact ** p)
+            assert abs(out[t, s] - n_ch * y) <= 1e-9 * max(1.0, abs(n_ch * y)), (s, t)
