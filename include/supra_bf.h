@@ -284,6 +284,21 @@ int supra_mv_run(const void* raw_delayed, int32_t in_dtype,
                  double sub_array_scaling_power, int32_t compute_means, double loading_factor,
                  void* out, int32_t out_dtype, void* stream);
 
+/* RxBeamformerMVpcg::performRxBeamforming (RxBeamformerMVpcg.cu:628-800),
+ * the BeamformingMVpcgNode engine: supra_mv_run's covariance, smoothing and
+ * loading (trace / L_t), with the weights from Jacobi-preconditioned
+ * conjugate gradients instead of a direct solve (solveBlockwisePCGJacobi,
+ * :450-546): x0 = 1, the residual recomputed every iteration, at most
+ * max_iterations_override iterations (0: the sub-array size, :649), stopping
+ * once |1 - R~ w| <= convergence_threshold (the node's
+ * 10^convergenceThresholdExponent).  Same layouts and types as supra_mv_run. */
+int supra_mvpcg_run(const void* raw_delayed, int32_t in_dtype,
+                    uint32_t num_scanlines, uint32_t num_channels, uint32_t num_samples,
+                    uint32_t sub_array_size, uint32_t temporal_smoothing,
+                    uint32_t max_iterations_override, double convergence_threshold,
+                    double sub_array_scaling_power, int32_t compute_means,
+                    void* out, int32_t out_dtype, void* stream);
+
 /* The pre-delay of supra_rawdelay_run feeding supra_mv_run, fused into one
  * launch: the pre-delayed samples are computed in LDS per workgroup (tile of
  * depths plus the temporal-smoothing halo) and never written to memory.
@@ -396,14 +411,15 @@ int supra_mock_read_frames(const char* raw_path, const supra_mock_meta* meta, ui
  * (src/SupraLib/Beamformer/HilbertFirEnvelope.cu:28-104; filter of
  * FirFilterFactory.h:55-200 with FilterTypeHilbertTransformer and
  * FilterWindowHamming; HilbertFirEnvelopeNode.cpp:42: filterLength 1..257,
- * default 65).  image/out: device [num_samples][num_scanlines], scanline
- * fastest (a USImage).  in_dtype SUPRA_INT16 or SUPRA_FLOAT32, out_dtype
+ * default 65).  image/out: device [num_frames][num_samples][num_scanlines],
+ * scanline fastest (num_frames USImages of one size, each filtered on its
+ * own; at most 65535 per call).  in_dtype SUPRA_INT16 or SUPRA_FLOAT32, out_dtype
  * SUPRA_INT16 or SUPRA_FLOAT32.  out = clampCast(sqrt(x^2 + (h * x)^2)), the
  * FIR over samples t - N/2 .. t + N/2 inside the image.  For an even N the
  * reference reads one coefficient past its filter; here that tap adds
  * nothing. */
 int supra_envelope_run(const void* image, int32_t in_dtype, uint32_t num_scanlines, uint32_t num_samples,
-                       uint32_t filter_length, void* out, int32_t out_dtype, void* stream);
+                       uint32_t num_frames, uint32_t filter_length, void* out, int32_t out_dtype, void* stream);
 
 /* LogCompressor::compress<InputType, OutputType>
  * (src/SupraLib/Beamformer/LogCompressor.cu:20-64; LogCompressorNode.cpp:42-45:
@@ -422,8 +438,9 @@ int supra_logcompress_run(const void* in, int32_t in_dtype, uint64_t count, doub
  * chains the two nodes through a device buffer).  out_dtype SUPRA_UINT8 or
  * SUPRA_FLOAT32; the result is bit-identical to the two calls. */
 int supra_envelope_logcompress_run(const void* image, int32_t in_dtype, uint32_t num_scanlines,
-                                   uint32_t num_samples, uint32_t filter_length, double dynamic_range_db,
-                                   double gain, double in_max, void* out, int32_t out_dtype, void* stream);
+                                   uint32_t num_samples, uint32_t num_frames, uint32_t filter_length,
+                                   double dynamic_range_db, double gain, double in_max,
+                                   void* out, int32_t out_dtype, void* stream);
 
 /* Window LUT of the reference (WindowFunction.cpp:18-47, WindowFunction.h:122-138),
  * computed on the host: SUPRA_WINDOW_NUM_ENTRIES floats, max-normalised. */

@@ -6,17 +6,18 @@
 //   LogCompressor (src/SupraLib/Beamformer/LogCompressor.cu:20-64,
 //     LogCompressorNode.cpp:42-45).
 //
-// Both are bandwidth-bound (the envelope reads each image sample N times, all
-// from LDS).  Envelope layout: one wave per (64 scanlines, 64 output
-// samples) of an image [sample][scanline]; lane = scanline, so every row of
-// the window is one coalesced 256-byte load, and the wave stages its column
-// window (64 + N - 1 rows) in LDS.  Samples outside the image are staged as
-// 0, which adds +-0 to the FIR sum where the reference skips them (the same
-// value).  The FIR sum runs over the taps in ascending order in single
-// precision without contraction, as the host restatement does, so the
-// envelope is bit for bit the oracle's.  The fused variant applies the log
-// compression to the float envelope in registers (the same operations as the
-// two-step path with a float intermediate).
+// Envelope layout: one workgroup of 4 waves per (64 scanlines, 32 output
+// samples, frame) of images [frame][sample][scanline]; lane = scanline, so
+// every row of the window is one coalesced 256-byte load.  The workgroup
+// stages the column window (32 + N - 1 rows) in LDS, loads issued eight
+// rows at a time per wave before they are written, and each wave then
+// computes every fourth output row from LDS.  Samples outside the image are
+// staged as 0, which adds +-0 to the FIR sum where the reference skips them
+// (the same value).  The FIR sum runs over the taps in ascending order in
+// single precision without contraction, as the host restatement does, so
+// the envelope is bit for bit the oracle's.  The fused variant applies the
+// log compression to the float envelope in registers (the same operations
+// as the two-step path with a float intermediate).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +32,8 @@
 namespace supra_amd {
 namespace {
 
-constexpr int kEnvRows = 64;          // output samples per wave
+constexpr int kEnvRows = 32;          // output samples per workgroup
+constexpr int kEnvWaves = 4;          // waves per workgroup
 constexpr int kMaxTaps = 257;         // HilbertFirEnvelopeNode's filterLength range (1..257)
 
 struct EnvArgs {
@@ -71,36 +73,46 @@ __device__ inline float log_compress(float a, float in_scale, float scale_den) {
 }
 
 template <typename InT, typename OutT, bool LOG>
-__global__ __launch_bounds__(64) void envelope_kernel(const EnvArgs a) {
-    extern __shared__ float win[];                 // [64 + 2 half rows][64 lanes]
-    const uint32_t lane = threadIdx.x;
+__global__ __launch_bounds__(64 * kEnvWaves) void envelope_kernel(const EnvArgs a) {
+    extern __shared__ float win[];                 // [32 + 2 half rows][64 lanes]
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t s = blockIdx.x * 64u + lane;
     const uint32_t t0 = blockIdx.y * kEnvRows;
+    const uint64_t frame = static_cast<uint64_t>(blockIdx.z) * a.n_s * a.n_rx;
     const uint32_t rows = kEnvRows + 2u * a.half;
     const bool col = s < a.n_rx;
-    for (uint32_t r = 0; r < rows; r++) {
-        const int64_t c = static_cast<int64_t>(t0) + r - a.half;
-        float v = 0.0f;
-        if (col && c >= 0 && c < static_cast<int64_t>(a.n_s))
-            v = load_as_float<InT>(a.in, static_cast<uint64_t>(c) * a.n_rx + s);
-        win[r * 64u + lane] = v;
+    const InT* in = static_cast<const InT*>(a.in) + frame;
+    for (uint32_t r0 = wave; r0 < rows; r0 += 8u * kEnvWaves) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t r = r0 + u * kEnvWaves;
+            const int64_t c = static_cast<int64_t>(t0) + r - a.half;
+            v[u] = 0.0f;
+            if (r < rows && col && c >= 0 && c < static_cast<int64_t>(a.n_s))
+                v[u] = static_cast<float>(in[static_cast<uint64_t>(c) * a.n_rx + s]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t r = r0 + u * kEnvWaves;
+            if (r < rows) win[r * 64u + lane] = v[u];
+        }
     }
-    // each lane reads only its own column: no barrier needed beyond the wave
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __syncthreads();
     if (!col) return;
-    for (uint32_t i = 0; i < kEnvRows; i++) {
+    OutT* out = static_cast<OutT*>(a.out) + frame;
+    for (uint32_t i = wave; i < kEnvRows; i += kEnvWaves) {
         const uint32_t t = t0 + i;
         if (t >= a.n_s) break;
         float acc = 0.0f;
+#pragma unroll 8
         for (uint32_t j = 0; j < a.used_taps; j++) acc += win[(i + j) * 64u + lane] * a.h[j];
         const float v = win[(i + a.half) * 64u + lane];
         float env = sqrtf(v * v + acc * acc);
         // the fused path sees the envelope as the float image the two-node
         // chain passes on: clampCast<float> first (NaN -> -FLT_MAX, inf -> FLT_MAX)
         if (LOG) env = log_compress(clamp_to<float>(env), a.in_scale, a.scale_den);
-        static_cast<OutT*>(a.out)[static_cast<uint64_t>(t) * a.n_rx + s] = clamp_to<OutT>(env);
+        out[static_cast<uint64_t>(t) * a.n_rx + s] = clamp_to<OutT>(env);
     }
 }
 
@@ -149,15 +161,16 @@ int check_types(const char* what, int in_dtype, int out_dtype, bool allow_u8_in,
 
 template <bool LOG, typename InT, typename OutT>
 void launch_env_typed(const EnvArgs& a, dim3 grid, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((envelope_kernel<InT, OutT, LOG>), grid, dim3(64), lds, s, a);
+    hipLaunchKernelGGL((envelope_kernel<InT, OutT, LOG>), grid, dim3(64 * kEnvWaves), lds, s, a);
 }
 
 template <bool LOG>
 int launch_envelope(const char* what, const void* image, int in_dtype, uint32_t n_rx, uint32_t n_s,
-                    uint32_t filter_length, float in_scale, float scale_den, void* out, int out_dtype,
-                    void* stream) {
+                    uint32_t n_frames, uint32_t filter_length, float in_scale, float scale_den, void* out,
+                    int out_dtype, void* stream) {
     if (!image || !out) return fail(SUPRA_ERR_INVALID_ARG, "%s: null argument", what);
-    if (n_rx == 0 || n_s == 0) return SUPRA_OK;
+    if (n_frames > 65535u) return fail(SUPRA_ERR_TOO_LARGE, "%s: %u frames in one call (at most 65535)", what, n_frames);
+    if (n_rx == 0 || n_s == 0 || n_frames == 0) return SUPRA_OK;
     if (filter_length < 1 || filter_length > kMaxTaps)
         return fail(SUPRA_ERR_INVALID_ARG, "%s: filter length %u outside [1, %d]", what, filter_length, kMaxTaps);
     int st = check_types(what, in_dtype, out_dtype, false, !LOG, LOG);
@@ -175,7 +188,7 @@ int launch_envelope(const char* what, const void* image, int in_dtype, uint32_t 
     a.in_scale = in_scale;
     a.scale_den = scale_den;
     hilbert_taps(filter_length, a.h);
-    const dim3 grid((n_rx + 63) / 64, (n_s + kEnvRows - 1) / kEnvRows);
+    const dim3 grid((n_rx + 63) / 64, (n_s + kEnvRows - 1) / kEnvRows, n_frames);
     if (grid.y > 65535u) return fail(SUPRA_ERR_TOO_LARGE, "%s: too many samples per scanline", what);
     const size_t lds = static_cast<size_t>(kEnvRows + 2 * a.half) * 64 * sizeof(float);
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -205,10 +218,11 @@ void launch_log_typed(const LogArgs& a, hipStream_t s) {
 using namespace supra_amd;
 
 extern "C" int supra_envelope_run(const void* image, int32_t in_dtype, uint32_t num_scanlines, uint32_t num_samples,
-                                  uint32_t filter_length, void* out, int32_t out_dtype, void* stream) {
+                                  uint32_t num_frames, uint32_t filter_length, void* out, int32_t out_dtype,
+                                  void* stream) {
     clear_error();
-    return launch_envelope<false>("envelope", image, in_dtype, num_scanlines, num_samples, filter_length, 0.0f, 0.0f,
-                                  out, out_dtype, stream);
+    return launch_envelope<false>("envelope", image, in_dtype, num_scanlines, num_samples, num_frames, filter_length,
+                                  0.0f, 0.0f, out, out_dtype, stream);
 }
 
 extern "C" int supra_logcompress_run(const void* in, int32_t in_dtype, uint64_t count, double dynamic_range_db,
@@ -234,12 +248,12 @@ extern "C" int supra_logcompress_run(const void* in, int32_t in_dtype, uint64_t 
 }
 
 extern "C" int supra_envelope_logcompress_run(const void* image, int32_t in_dtype, uint32_t num_scanlines,
-                                              uint32_t num_samples, uint32_t filter_length, double dynamic_range_db,
-                                              double gain, double in_max, void* out, int32_t out_dtype,
-                                              void* stream) {
+                                              uint32_t num_samples, uint32_t num_frames, uint32_t filter_length,
+                                              double dynamic_range_db, double gain, double in_max, void* out,
+                                              int32_t out_dtype, void* stream) {
     clear_error();
     float in_scale, scale_den;
     log_constants(SUPRA_FLOAT32, dynamic_range_db, gain, in_max, &in_scale, &scale_den);
-    return launch_envelope<true>("envelope + logcompress", image, in_dtype, num_scanlines, num_samples,
+    return launch_envelope<true>("envelope + logcompress", image, in_dtype, num_scanlines, num_samples, num_frames,
                                  filter_length, in_scale, scale_den, out, out_dtype, stream);
 }

@@ -34,10 +34,17 @@ def _code(dtype) -> int:
 
 
 def _image(img):
+    """Device tensor [sample][scanline] or a batch [frame][sample][scanline]."""
     data = img.data if isinstance(img, USImage) else img
-    if data.dim() != 2 or not data.is_cuda or not data.is_contiguous():
-        raise ValueError("image must be a contiguous device tensor [sample][scanline]")
+    if data.dim() not in (2, 3) or not data.is_cuda or not data.is_contiguous():
+        raise ValueError("image must be a contiguous device tensor [sample][scanline] "
+                         "or [frame][sample][scanline]")
     return data
+
+
+def _dims(data):
+    """(frames, samples, scanlines) of an image or batch."""
+    return (1, *(int(v) for v in data.shape)) if data.dim() == 2 else tuple(int(v) for v in data.shape)
 
 
 class HilbertFirEnvelope:
@@ -49,12 +56,12 @@ class HilbertFirEnvelope:
 
     def demodulate(self, image, out_dtype=torch.float32, out=None, stream=None) -> USImage:
         data = _image(image)
-        n_s, n_rx = (int(v) for v in data.shape)
+        n_f, n_s, n_rx = _dims(data)
         if out is None:
-            out = torch.empty((n_s, n_rx), dtype=out_dtype, device=data.device)
-        elif tuple(out.shape) != (n_s, n_rx) or out.dtype != out_dtype or not out.is_contiguous():
-            raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {(n_s, n_rx)}")
-        N.check(N.lib().supra_envelope_run(C.c_void_p(data.data_ptr()), _code(data.dtype), n_rx, n_s,
+            out = torch.empty(tuple(data.shape), dtype=out_dtype, device=data.device)
+        elif tuple(out.shape) != tuple(data.shape) or out.dtype != out_dtype or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {tuple(data.shape)}")
+        N.check(N.lib().supra_envelope_run(C.c_void_p(data.data_ptr()), _code(data.dtype), n_rx, n_s, n_f,
                                            self.filter_length, C.c_void_p(out.data_ptr()), _code(out_dtype),
                                            C.c_void_p(_stream_handle(stream))))
         props = dict(image.properties) if isinstance(image, USImage) else {}
@@ -89,12 +96,12 @@ def envelope_log_compress(image, filter_length: int = 65, dynamic_range: float =
                           in_max: float = 1024.0, out_dtype=torch.uint8, out=None, stream=None) -> USImage:
     """HilbertFirEnvelope (float envelope) then LogCompressor in one launch."""
     data = _image(image)
-    n_s, n_rx = (int(v) for v in data.shape)
+    n_f, n_s, n_rx = _dims(data)
     if out is None:
-        out = torch.empty((n_s, n_rx), dtype=out_dtype, device=data.device)
-    elif tuple(out.shape) != (n_s, n_rx) or out.dtype != out_dtype or not out.is_contiguous():
-        raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {(n_s, n_rx)}")
-    N.check(N.lib().supra_envelope_logcompress_run(C.c_void_p(data.data_ptr()), _code(data.dtype), n_rx, n_s,
+        out = torch.empty(tuple(data.shape), dtype=out_dtype, device=data.device)
+    elif tuple(out.shape) != tuple(data.shape) or out.dtype != out_dtype or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous {out_dtype} tensor of shape {tuple(data.shape)}")
+    N.check(N.lib().supra_envelope_logcompress_run(C.c_void_p(data.data_ptr()), _code(data.dtype), n_rx, n_s, n_f,
                                                    int(filter_length), float(dynamic_range), float(scale),
                                                    float(in_max), C.c_void_p(out.data_ptr()), _code(out_dtype),
                                                    C.c_void_p(_stream_handle(stream))))

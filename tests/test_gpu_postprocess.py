@@ -108,11 +108,40 @@ def test_postprocess_rejects_bad_arguments(device):
     out = torch.zeros((10, 4), device="cuda")
     lib = N.lib()
     import ctypes as C
-    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_FLOAT32, 4, 10, 0, C.c_void_p(out.data_ptr()),
+    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_FLOAT32, 4, 10, 1, 0, C.c_void_p(out.data_ptr()),
                                   N.SUPRA_FLOAT32, None) == N.SUPRA_ERR_INVALID_ARG
-    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_FLOAT32, 4, 10, 258, C.c_void_p(out.data_ptr()),
+    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_FLOAT32, 4, 10, 1, 258, C.c_void_p(out.data_ptr()),
                                   N.SUPRA_FLOAT32, None) == N.SUPRA_ERR_INVALID_ARG
-    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_UINT8, 4, 10, 65, C.c_void_p(out.data_ptr()),
+    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_UINT8, 4, 10, 1, 65, C.c_void_p(out.data_ptr()),
                                   N.SUPRA_FLOAT32, None) == N.SUPRA_ERR_UNSUPPORTED
     assert lib.supra_logcompress_run(C.c_void_p(x.data_ptr()), N.SUPRA_FLOAT32, 40, 80.0, 1.0, 1024.0,
                                      C.c_void_p(out.data_ptr()), N.SUPRA_INT16, None) == N.SUPRA_ERR_UNSUPPORTED
+    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_FLOAT32, 4, 10, 70000, 65,
+                                  C.c_void_p(out.data_ptr()), N.SUPRA_FLOAT32, None) == N.SUPRA_ERR_TOO_LARGE
+    assert lib.supra_envelope_run(C.c_void_p(x.data_ptr()), N.SUPRA_FLOAT32, 4, 10, 0, 65,
+                                  C.c_void_p(out.data_ptr()), N.SUPRA_FLOAT32, None) == N.SUPRA_OK
+
+
+@pytest.mark.parametrize("n_taps,fused", [(65, False), (64, True), (17, True)])
+def test_envelope_frame_batch_equals_single_frames(device, n_taps, fused):
+    """A batch of frames in one launch: each frame is its own image (the FIR
+    does not cross frames), bit for bit the single-frame call and the oracle."""
+    from supra_amd.postprocess import HilbertFirEnvelope, envelope_log_compress
+    rng = np.random.default_rng(n_taps)
+    x = (rng.normal(size=(5, 203, 150)) * 3000).astype(np.float32)
+    dev = torch.from_numpy(x).cuda()
+    if fused:
+        batch = envelope_log_compress(dev, n_taps, 70.0, 1.2, 9000.0, out_dtype=torch.float32).data
+        singles = [envelope_log_compress(dev[f], n_taps, 70.0, 1.2, 9000.0, out_dtype=torch.float32).data
+                   for f in range(5)]
+    else:
+        batch = HilbertFirEnvelope(n_taps).demodulate(dev, torch.float32).data
+        singles = [HilbertFirEnvelope(n_taps).demodulate(dev[f], torch.float32).data for f in range(5)]
+    torch.cuda.synchronize()
+    got = batch.cpu().numpy()
+    assert got.shape == x.shape
+    for f in range(5):
+        assert np.array_equal(got[f], singles[f].cpu().numpy())
+        env = O.clamp_cast(O.envelope(x[f], n_taps), np.float32)
+        if not fused:
+            assert np.array_equal(got[f], env)
