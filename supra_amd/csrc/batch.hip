@@ -11,7 +11,8 @@
 // kernel sized to the GPU; workgroups take items in frame order, so frames
 // complete in submission order within a queue, and the last item of a frame
 // stamps the frame's completion time from the device wall clock.  The queues
-// run concurrently (a fork/join over one stream per queue) and the whole
+// run concurrently (a fork/join over one stream per queue) or one after
+// another (see the autotuner below), and the whole
 // launch — reset of the queue counters, the kernels — is captured once in a
 // hipGraph; supra_batch_launch replays it.
 //
@@ -22,7 +23,11 @@
 //
 // The autotuner times each queue alone on the real batch for a few
 // candidates (DAS: persistent workgroups per CU; MV: depth tile and LDS
-// budget, i.e. one or two workgroups per CU) and keeps the fastest.
+// budget, i.e. one or two workgroups per CU) and keeps the fastest; then it
+// times the whole launch with the queues concurrent and with the queues one
+// after another (shortest time per frame first) and keeps the second unless
+// the first is more than 5 % faster.  Without the autotuner the queues run
+// concurrently.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -76,6 +81,7 @@ struct Queue {
     uint32_t n_items = 0, n_frames = 0;
     size_t lds = 0, budget = 0;
     uint32_t wg_per_cu = 1;
+    float alone_ms = 0;        // autotuned time of the queue alone (0: not timed)
     hipStream_t stream = nullptr;
     hipEvent_t joined = nullptr;
 };
@@ -97,6 +103,10 @@ struct supra_batch {
     uint32_t* heads = nullptr;
     uint32_t* done = nullptr;
     std::string config;
+    // schedule: the queues concurrently (fork/join), or one after another
+    // in `order` on one stream (chosen by the autotuner)
+    bool ordered = false;
+    std::vector<uint32_t> order;
     // graph
     hipStream_t origin = nullptr;      // capture stream: reset, fork, join
     hipEvent_t ev_fork = nullptr;
@@ -214,6 +224,9 @@ int upload(D** dst, const std::vector<T>& src, const char* what) {
 }
 
 int build_das_queue(supra_batch* b, uint32_t qi) {
+    if (das_batch_frame_bytes() != sizeof(DasBatchFrame))
+        return fail(SUPRA_ERR_INVALID_ARG, "batch: DAS frame layout %zu bytes here, %zu in the kernels: the library "
+                    "links objects of different revisions, rebuild it", sizeof(DasBatchFrame), das_batch_frame_bytes());
     Queue& q = b->queues[qi];
     std::vector<DasBatchFrame> frames;
     std::vector<uint32_t> items;
@@ -243,6 +256,9 @@ int build_das_queue(supra_batch* b, uint32_t qi) {
 
 // Plans the MV frames of queue qi for an LDS budget.
 int build_mv_queue(supra_batch* b, uint32_t qi, size_t budget) {
+    if (mv_batch_frame_bytes() != sizeof(MvBatchFrame))
+        return fail(SUPRA_ERR_INVALID_ARG, "batch: MV frame layout %zu bytes here, %zu in the kernels: the library "
+                    "links objects of different revisions, rebuild it", sizeof(MvBatchFrame), mv_batch_frame_bytes());
     Queue& q = b->queues[qi];
     std::vector<MvBatchFrame> frames;
     std::vector<uint32_t> items;
@@ -354,6 +370,72 @@ int time_queue(supra_batch* b, uint32_t q, float* ms) {
     return SUPRA_OK;
 }
 
+// One launch of every queue (after a reset) on the origin stream: all
+// concurrently through the fork/join streams, or one after another in
+// b->order.  Returns the time of the best of two runs after a warm-up.
+int time_schedule(supra_batch* b, bool ordered, float* ms) {
+    hipStream_t s0 = b->origin;
+    hipEvent_t e0, e1;
+    hipError_t e;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(e, "batch: event");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(e, "batch: event");
+    }
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+        if ((e = launch_reset(b, s0)) != hipSuccess) break;
+        if ((e = hipEventRecord(e0, s0)) != hipSuccess) break;
+        if (ordered) {
+            for (uint32_t qi : b->order)
+                if ((e = launch_queue(b, qi, s0)) != hipSuccess) break;
+        } else {
+            if ((e = hipEventRecord(b->ev_fork, s0)) != hipSuccess) break;
+            for (uint32_t qi = 0; qi < b->queues.size() && e == hipSuccess; qi++) {
+                e = hipStreamWaitEvent(b->queues[qi].stream, b->ev_fork, 0);
+                if (e == hipSuccess) e = launch_queue(b, qi, b->queues[qi].stream);
+                if (e == hipSuccess) e = hipEventRecord(b->queues[qi].joined, b->queues[qi].stream);
+                if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->queues[qi].joined, 0);
+            }
+        }
+        if (e != hipSuccess) break;
+        if ((e = hipEventRecord(e1, s0)) != hipSuccess) break;
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+        float t = 0;
+        if ((e = hipEventElapsedTime(&t, e0, e1)) != hipSuccess) break;
+        if (rep > 0) best = std::min(best, t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "batch: schedule run");
+    *ms = best;
+    return SUPRA_OK;
+}
+
+// The order that minimises the mean frame latency when the queues run one
+// after another (Smith's rule: ascending queue time per frame), and the
+// schedule: one after another unless running the queues concurrently is
+// more than 5 % faster.  One after another a short queue's frames finish as
+// soon as it does instead of sharing the GPU with the long queues.
+int choose_schedule(supra_batch* b) {
+    b->order.clear();
+    for (uint32_t qi = 0; qi < b->queues.size(); qi++)
+        if (b->queues[qi].n_items) b->order.push_back(qi);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
+        const Queue &qx = b->queues[x], &qy = b->queues[y];
+        return qx.alone_ms * qy.n_frames < qy.alone_ms * qx.n_frames;
+    });
+    float t_conc = 0, t_ord = 0;
+    int st = time_schedule(b, false, &t_conc);
+    if (st == SUPRA_OK) st = time_schedule(b, true, &t_ord);
+    if (st != SUPRA_OK) return st;
+    b->ordered = t_ord <= 1.05f * t_conc;
+    char line[160];
+    std::snprintf(line, sizeof(line), "concurrent %.3f ms; in order %.3f ms; ", t_conc, t_ord);
+    b->config += line;
+    return SUPRA_OK;
+}
+
 int autotune(supra_batch* b) {
     char line[200];
     int st;
@@ -376,6 +458,7 @@ int autotune(supra_batch* b) {
                 }
             }
             q.wg_per_cu = pick;
+            q.alone_ms = best;
             continue;
         }
         struct Cand { size_t budget; uint32_t wg; };
@@ -397,9 +480,10 @@ int autotune(supra_batch* b) {
         }
         clear_error();
         q.wg_per_cu = pick.wg;
+        q.alone_ms = best;
         if ((st = build_mv_queue(b, qi, pick.budget)) != SUPRA_OK) return st;
     }
-    return SUPRA_OK;
+    return choose_schedule(b);
 }
 
 int capture_graph(supra_batch* b) {
@@ -408,12 +492,17 @@ int capture_graph(supra_batch* b) {
     if ((e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return hip_fail(e, "batch: begin capture");
     e = launch_reset(b, s0);
-    if (e == hipSuccess) e = hipEventRecord(b->ev_fork, s0);
-    for (uint32_t q = 0; q < b->queues.size() && e == hipSuccess; q++) {
-        e = hipStreamWaitEvent(b->queues[q].stream, b->ev_fork, 0);
-        if (e == hipSuccess) e = launch_queue(b, q, b->queues[q].stream);
-        if (e == hipSuccess) e = hipEventRecord(b->queues[q].joined, b->queues[q].stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->queues[q].joined, 0);
+    if (b->ordered) {
+        for (uint32_t q : b->order)
+            if (e == hipSuccess) e = launch_queue(b, q, s0);
+    } else {
+        if (e == hipSuccess) e = hipEventRecord(b->ev_fork, s0);
+        for (uint32_t q = 0; q < b->queues.size() && e == hipSuccess; q++) {
+            e = hipStreamWaitEvent(b->queues[q].stream, b->ev_fork, 0);
+            if (e == hipSuccess) e = launch_queue(b, q, b->queues[q].stream);
+            if (e == hipSuccess) e = hipEventRecord(b->queues[q].joined, b->queues[q].stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->queues[q].joined, 0);
+        }
     }
     hipGraph_t g = nullptr;
     const hipError_t e_end = hipStreamEndCapture(s0, &g);
@@ -532,7 +621,7 @@ extern "C" int supra_batch_create(const supra_batch_item* items, uint32_t num_it
     if (st != SUPRA_OK) return st;
     b->config.clear();
     if (autotune_tiles && (st = autotune(b)) != SUPRA_OK) return st;
-    b->config += "chosen:";
+    b->config += b->ordered ? "chosen: in order" : "chosen: concurrent";
     for (const Queue& q : b->queues) {
         char line[200];
         if (!q.mv)
@@ -542,6 +631,10 @@ extern "C" int supra_batch_create(const supra_batch_item* items, uint32_t num_it
                           q.rf_dtype == SUPRA_INT16 ? "int16" : "float", q.means ? " means" : "", q.n_frames,
                           q.n_items, q.wg_per_cu, q.lds);
         b->config += line;
+    }
+    if (b->ordered) {
+        b->config += " order:";
+        for (uint32_t q : b->order) b->config += " " + std::to_string(q);
     }
     if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(e, "batch: setup");
     if ((st = capture_graph(b)) != SUPRA_OK) return st;

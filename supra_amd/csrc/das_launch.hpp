@@ -25,6 +25,11 @@ hipError_t launch_rawdelay(const DasArgs& a, int rf_dtype, int out_dtype, bool i
 // Persistent work-queue DAS over the frames of a batch (das_kernels.hip).
 hipError_t launch_das_batch(const DasBatchArgs& b, uint32_t workgroups, hipStream_t stream);
 
+// sizeof(DasBatchFrame) as the kernels' translation unit sees it: a batch
+// checks its own view against it before it uploads a frame table (objects
+// built from different revisions of the struct must not be linked together).
+size_t das_batch_frame_bytes();
+
 // Differentiable 2D DAS (das_adjoint.hip).
 hipError_t launch_frame_tables(const FrameTablesArgs& a, hipStream_t stream);
 hipError_t launch_beta(const float* grad_image, const float* norm, float* beta, uint32_t n_rx, uint32_t n_d,
