@@ -922,6 +922,8 @@ static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool i
 
 uint32_t das_depth_tile() { return kThreads; }
 
+size_t das_batch_frame_bytes() { return sizeof(DasBatchFrame); }
+
 hipError_t launch_das_batch(const DasBatchArgs& b, uint32_t workgroups, hipStream_t stream) {
     hipLaunchKernelGGL(das2d_batch_kernel, dim3(workgroups), dim3(kThreads), 0, stream, b);
     return hipGetLastError();
