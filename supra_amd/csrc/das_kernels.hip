@@ -142,19 +142,33 @@ __device__ inline Tile decode_tile(const DasArgs& a) {
     return t;
 }
 
-// The 2D kernel's unit: wave w of workgroup b takes unit b * 4 + w of the
-// (frame, scanline, 64-depth chunk) list; units past the end are invalid.
+// The hardware hands workgroup b to XCD b % 8 (round robin).  This
+// renumbering gives every XCD one contiguous range of the logical workgroup
+// index, so that workgroups whose outputs share cache lines run on the same
+// XCD and their partial lines merge in that XCD's L2 instead of reaching
+// HBM as separate partial writes.  A bijection on [0, nb).
+__device__ inline uint32_t xcd_contiguous(uint32_t b, uint32_t nb) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t xcd = b % kXcds, local = b / kXcds;
+    const uint32_t per = nb / kXcds, rem = nb % kXcds;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// The 2D kernels' unit: wave w of (renumbered) workgroup b takes unit
+// b * 4 + w of the (frame, 64-depth chunk, scanline) list, scanline fastest:
+// lanes are depths, so an output row (consecutive scanlines at one depth) is
+// written by neighbouring units; units past the end are invalid.
 __device__ inline Tile decode_wave_2d(const DasArgs& a) {
     const uint32_t chunks = (a.n_depths + kWaveDepths - 1) / kWaveDepths;
     const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
-    const uint64_t unit = static_cast<uint64_t>(blockIdx.x) * kWavesPerGroup + wave;
-    const uint64_t rest = unit / chunks;
+    const uint64_t unit = static_cast<uint64_t>(xcd_contiguous(blockIdx.x, gridDim.x)) * kWavesPerGroup + wave;
+    const uint64_t rest = unit / a.n_rx;
     Tile t;
-    t.s = static_cast<uint32_t>(rest % a.n_rx);
-    t.frame = static_cast<uint32_t>(rest / a.n_rx);
-    t.r0 = static_cast<uint32_t>(unit % chunks) * kWaveDepths;
+    t.s = static_cast<uint32_t>(unit % a.n_rx);
+    t.frame = static_cast<uint32_t>(rest / chunks);
+    t.r0 = static_cast<uint32_t>(rest % chunks) * kWaveDepths;
     t.r = t.r0 + threadIdx.x % 64;
-    t.valid = rest < static_cast<uint64_t>(a.n_rx) * a.n_frames && t.r < a.n_depths;
+    t.valid = rest < static_cast<uint64_t>(chunks) * a.n_frames && t.r < a.n_depths;
     return t;
 }
 
@@ -385,12 +399,14 @@ __global__ __launch_bounds__(kThreads) void das2d_frames_kernel(const DasArgs a)
     const int lane = static_cast<int>(threadIdx.x % 64);
     const uint32_t chunks = (a.n_depths + kWaveDepths - 1) / kWaveDepths;
     const uint32_t groups = (a.n_frames + FB - 1) / FB;
-    const uint64_t unit = static_cast<uint64_t>(blockIdx.x) * kWavesPerGroup + wave;
-    const uint64_t rest = unit / chunks;
-    const uint32_t s = static_cast<uint32_t>(rest % a.n_rx);
-    const uint64_t grp = rest / a.n_rx;
+    // (frame group, 64-depth chunk, scanline) units, scanline fastest, in
+    // XCD-contiguous workgroup order (decode_wave_2d)
+    const uint64_t unit = static_cast<uint64_t>(xcd_contiguous(blockIdx.x, gridDim.x)) * kWavesPerGroup + wave;
+    const uint64_t rest = unit / a.n_rx;
+    const uint32_t s = static_cast<uint32_t>(unit % a.n_rx);
+    const uint64_t grp = rest / chunks;
     if (grp >= groups) return;                       // (wave-uniform) past the end of the unit list
-    const uint32_t r0 = static_cast<uint32_t>(unit % chunks) * kWaveDepths;
+    const uint32_t r0 = static_cast<uint32_t>(rest % chunks) * kWaveDepths;
     const uint32_t r = r0 + static_cast<uint32_t>(lane);
     const bool valid = r < a.n_depths;
     const uint32_t f0 = static_cast<uint32_t>(grp) * FB;
