@@ -20,6 +20,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libsupra_bf.so"
+SHIM_TEST = PKG / "shim_test"
+TORCH_LIB = PKG / "libsupra_torch.so"
 
 ARCH = os.environ.get("SUPRA_OFFLOAD_ARCH", "gfx950")
 
@@ -70,7 +72,27 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)], verbose)
     os.replace(tmp, LIB)
+    # the C++ engine shim's test program (include/supra_bf.hpp), next to the library
+    _run([hipcc, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "shim_test.cpp"),
+          f"-L{PKG}", "-lsupra_bf", "-Wl,-rpath,$ORIGIN", "-o", str(SHIM_TEST)], verbose)
+    build_torch_op(hipcc, verbose)
     return LIB
+
+
+def build_torch_op(hipcc: str, verbose: bool) -> None:
+    """libsupra_torch.so: the libtorch autograd operator supra_bf::das_c
+    (include/supra_bf_torch.hpp, csrc/torch_op.cpp), host code against the
+    installed torch's headers and libraries."""
+    import torch
+    tdir = Path(torch.__file__).resolve().parent
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    tmp = TORCH_LIB.with_suffix(".so.tmp")
+    _run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1",
+          f"-D_GLIBCXX_USE_CXX11_ABI={abi}", f"-I{tdir / 'include'}", f"-I{tdir / 'include' / 'torch' / 'csrc' / 'api' / 'include'}",
+          f"-I{ROOT / 'include'}", str(CSRC / "torch_op.cpp"), f"-L{tdir / 'lib'}", "-ltorch", "-ltorch_cpu", "-lc10",
+          "-lc10_hip", "-ltorch_hip", f"-L{PKG}", "-lsupra_bf", f"-Wl,-rpath,{tdir / 'lib'}", "-Wl,-rpath,$ORIGIN",
+          "-o", str(tmp)], verbose)
+    os.replace(tmp, TORCH_LIB)
 
 
 if __name__ == "__main__":
