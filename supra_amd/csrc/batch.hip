@@ -22,8 +22,9 @@
 // each class at its own.
 //
 // The autotuner times each queue alone on the real batch for a few
-// candidates (DAS: persistent workgroups per CU; MV: depth tile and LDS
-// budget, i.e. one or two workgroups per CU) and keeps the fastest; then it
+// candidates (DAS: 2, 4, 7 or 8 persistent workgroups per CU; MV: one to
+// four workgroups per CU, each with the depth tile that fits its share of
+// the LDS) and keeps the fastest; then it
 // times the whole launch with the queues concurrent and with the queues one
 // after another (shortest time per frame first) and keeps the second unless
 // the first is more than 5 % faster.  Without the autotuner the queues run
@@ -303,8 +304,9 @@ size_t mv_full_budget(const supra_batch* b) {
     (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device);
     return static_cast<size_t>(dev_lds) - kMvStaticLdsReserve;
 }
-size_t mv_half_budget(const supra_batch* b) {
-    return (mv_full_budget(b) + kMvStaticLdsReserve) / 2 - kMvStaticLdsReserve;
+// The LDS budget of one of `wg` workgroups sharing a CU's LDS.
+size_t mv_share_budget(const supra_batch* b, uint32_t wg) {
+    return (mv_full_budget(b) + kMvStaticLdsReserve) / wg - kMvStaticLdsReserve;
 }
 
 BatchSync make_sync(const supra_batch* b, uint32_t q) {
@@ -443,7 +445,7 @@ int autotune(supra_batch* b) {
         Queue& q = b->queues[qi];
         if (!q.n_items) continue;
         if (!q.mv) {
-            const uint32_t cands[] = {2, 4, 7};
+            const uint32_t cands[] = {2, 4, 7, 8};
             float best = 1e30f;
             uint32_t pick = 4;
             for (uint32_t c : cands) {
@@ -462,7 +464,11 @@ int autotune(supra_batch* b) {
             continue;
         }
         struct Cand { size_t budget; uint32_t wg; };
-        const Cand cands[] = {{mv_full_budget(b), 1}, {mv_half_budget(b), 2}};
+        // one to four workgroups per CU, each planning its depth tile for
+        // its share of the LDS (a larger tile re-reads less of the temporal
+        // halo; more workgroups hide more latency)
+        const Cand cands[] = {{mv_full_budget(b), 1}, {mv_share_budget(b, 2), 2}, {mv_share_budget(b, 3), 3},
+                              {mv_share_budget(b, 4), 4}};
         float best = 1e30f;
         Cand pick = cands[0];
         for (const Cand& c : cands) {
