@@ -569,6 +569,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--frames-2d", type=int, default=32, help="frames per launch of the 2D DAS leg")
     ap.add_argument("--no-2d", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-mv", action="store_true")
@@ -607,7 +608,7 @@ def main() -> int:
             roofline["traffic"] = t["hbm_bytes_per_launch"]
     extra = {}
     if not args.no_2d:
-        extra["linear_2d"] = bench_2d(args, torch)
+        extra["linear_2d"] = bench_2d(args, torch, frames=args.frames_2d)
     if not args.no_diff:
         extra["diff_2d"] = bench_diff(args, torch)
     if not args.no_mv:

@@ -39,6 +39,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #include "bf_internal.hpp"
 #include "das_device.hpp"
@@ -142,33 +143,46 @@ __device__ inline Tile decode_tile(const DasArgs& a) {
     return t;
 }
 
-// The hardware hands workgroup b to XCD b % 8 (round robin).  This
-// renumbering gives every XCD one contiguous range of the logical workgroup
-// index, so that workgroups whose outputs share cache lines run on the same
-// XCD and their partial lines merge in that XCD's L2 instead of reaching
-// HBM as separate partial writes.  A bijection on [0, nb).
-__device__ inline uint32_t xcd_contiguous(uint32_t b, uint32_t nb) {
-    constexpr uint32_t kXcds = 8;
+// The hardware deals workgroup b to XCD b % 8 (round robin), and each XCD
+// has its own L2.  2D workgroups are renumbered in tiles of kXcdTileWgs
+// consecutive logical workgroups (32 units: 32 neighbouring scanlines of
+// one 64-depth chunk, which write the same output cache lines), dealt to
+// the XCDs in rounds of 8 tiles: round j gives tile 8j + (x + j) % 8 to XCD
+// x.  Each tile runs whole on one XCD, so the partial lines of an output
+// row merge in its L2 instead of reaching HBM as separate writes, and the
+// rotation spreads shallow and deep, central and edge tiles over all XCDs
+// (a contiguous range per XCD left half of them with the shallow chunks and
+// 27 % less work than the others).  A bijection on [0, gridDim.x): the
+// launch pads the grid to whole rounds, padded units are past the end.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdTileWgs = 8;
+__device__ inline uint32_t logical_wg_2d() {
+    const uint32_t b = blockIdx.x;
     const uint32_t xcd = b % kXcds, local = b / kXcds;
-    const uint32_t per = nb / kXcds, rem = nb % kXcds;
-    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+    const uint32_t round = local / kXcdTileWgs, k = local % kXcdTileWgs;
+    return (round * kXcds + (xcd + round) % kXcds) * kXcdTileWgs + k;
 }
 
 // The 2D kernels' unit: wave w of (renumbered) workgroup b takes unit
-// b * 4 + w of the (frame, 64-depth chunk, scanline) list, scanline fastest:
+// b * 4 + w of the (64-depth chunk, frame, scanline) list, scanline fastest:
 // lanes are depths, so an output row (consecutive scanlines at one depth) is
-// written by neighbouring units; units past the end are invalid.
+// written by neighbouring units.  Chunks run from the deepest to the
+// shallowest: deep units have the widest aperture and take longest, and
+// dispatching them first leaves short units for the end of the grid.  Units
+// past the end are invalid.
 __device__ inline Tile decode_wave_2d(const DasArgs& a) {
     const uint32_t chunks = (a.n_depths + kWaveDepths - 1) / kWaveDepths;
     const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64)));
-    const uint64_t unit = static_cast<uint64_t>(xcd_contiguous(blockIdx.x, gridDim.x)) * kWavesPerGroup + wave;
+    const uint64_t unit = static_cast<uint64_t>(logical_wg_2d()) * kWavesPerGroup + wave;
     const uint64_t rest = unit / a.n_rx;
     Tile t;
     t.s = static_cast<uint32_t>(unit % a.n_rx);
-    t.frame = static_cast<uint32_t>(rest / chunks);
-    t.r0 = static_cast<uint32_t>(rest % chunks) * kWaveDepths;
+    const uint64_t cr = rest / a.n_frames;            // chunk, counted from the deepest
+    const bool in_list = cr < chunks;
+    t.frame = static_cast<uint32_t>(rest % a.n_frames);
+    t.r0 = (in_list ? chunks - 1u - static_cast<uint32_t>(cr) : 0u) * kWaveDepths;
     t.r = t.r0 + threadIdx.x % 64;
-    t.valid = rest < static_cast<uint64_t>(chunks) * a.n_frames && t.r < a.n_depths;
+    t.valid = in_list && t.r < a.n_depths;
     return t;
 }
 
@@ -399,14 +413,15 @@ __global__ __launch_bounds__(kThreads) void das2d_frames_kernel(const DasArgs a)
     const int lane = static_cast<int>(threadIdx.x % 64);
     const uint32_t chunks = (a.n_depths + kWaveDepths - 1) / kWaveDepths;
     const uint32_t groups = (a.n_frames + FB - 1) / FB;
-    // (frame group, 64-depth chunk, scanline) units, scanline fastest, in
-    // XCD-contiguous workgroup order (decode_wave_2d)
-    const uint64_t unit = static_cast<uint64_t>(xcd_contiguous(blockIdx.x, gridDim.x)) * kWavesPerGroup + wave;
+    // (64-depth chunk from the deepest, frame group, scanline) units,
+    // scanline fastest, in XCD tiles (decode_wave_2d)
+    const uint64_t unit = static_cast<uint64_t>(logical_wg_2d()) * kWavesPerGroup + wave;
     const uint64_t rest = unit / a.n_rx;
     const uint32_t s = static_cast<uint32_t>(unit % a.n_rx);
-    const uint64_t grp = rest / chunks;
-    if (grp >= groups) return;                       // (wave-uniform) past the end of the unit list
-    const uint32_t r0 = static_cast<uint32_t>(rest % chunks) * kWaveDepths;
+    const uint64_t cr = rest / groups;               // chunk, counted from the deepest
+    if (cr >= chunks) return;                        // (wave-uniform) past the end of the unit list
+    const uint64_t grp = rest % groups;
+    const uint32_t r0 = (chunks - 1u - static_cast<uint32_t>(cr)) * kWaveDepths;
     const uint32_t r = r0 + static_cast<uint32_t>(lane);
     const bool valid = r < a.n_depths;
     const uint32_t f0 = static_cast<uint32_t>(grp) * FB;
@@ -898,12 +913,18 @@ __global__ __launch_bounds__(kThreads) void das2d_batch_kernel(const DasBatchArg
 
 }  // namespace
 
+// 2D grids are padded to whole rounds of XCD tiles (logical_wg_2d).
+static uint64_t pad_2d_grid(uint64_t blocks) {
+    const uint64_t round = static_cast<uint64_t>(kXcds) * kXcdTileWgs;
+    return (blocks + round - 1) / round * round;
+}
+
 template <typename RFT, typename OutT>
 static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool interp_tx, bool map,
                                hipStream_t stream) {
     const uint64_t units2d = static_cast<uint64_t>((a.n_depths + kWaveDepths - 1) / kWaveDepths) * a.n_rx * a.n_frames;
     const uint64_t blocks = is3d ? static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames
-                                 : (units2d + kWavesPerGroup - 1) / kWavesPerGroup;
+                                 : pad_2d_grid((units2d + kWavesPerGroup - 1) / kWavesPerGroup);
     dim3 grid(static_cast<uint32_t>(blocks)), block(kThreads);
 #define SUPRA_DAS_DISPATCH(KERNEL)                                                                                 \
     do {                                                                                                            \
@@ -926,7 +947,7 @@ static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool i
         constexpr int FB = kFramesPerWave;
         const uint64_t units = static_cast<uint64_t>((a.n_depths + kWaveDepths - 1) / kWaveDepths) * a.n_rx *
                                ((a.n_frames + FB - 1) / FB);
-        dim3 g2(static_cast<uint32_t>((units + kWavesPerGroup - 1) / kWavesPerGroup));
+        dim3 g2(static_cast<uint32_t>(pad_2d_grid((units + kWavesPerGroup - 1) / kWavesPerGroup)));
         if (map) hipLaunchKernelGGL((das2d_frames_kernel<RFT, OutT, FB, true>), g2, block, 0, stream, a);
         else hipLaunchKernelGGL((das2d_frames_kernel<RFT, OutT, FB, false>), g2, block, 0, stream, a);
     } else {
