@@ -563,12 +563,31 @@ __device__ inline void gather3d_consume(const Gather3D<RFT>& g, float& sample) {
 // latency.  Weights, counts and samples are still added in element order.
 constexpr int kBatch3D = 4;   // 2: 5.63 ms, 4: 4.9-5.0 ms, 8: 5.09 ms, 16: 5.27 ms per C1 volume
 
+// Per-row values of the element loop: elements come x outer, y inner, so
+// consecutive elements of one x column share xe, hence (xe - px)^2 and the x
+// window index.  They are recomputed whenever the staged xe changes (a
+// wave-uniform test), so the values are the same operations on the same
+// inputs as per element, for any element layout.
+struct Row3D {
+    uint32_t xe_bits;
+    float ax2;
+    int ix;
+};
+
 template <typename RFT, bool FAR, bool MAP>
 __device__ inline Gather3D<RFT> element3d_issue(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float t0, float offset,
                                                 const Pixel3D& p, const float* s_xe, const float* s_ze,
                                                 const float* s_dx, const float* s_dz, const float* s_r2,
                                                 const float* s_w, const uint32_t* s_off, const float* s_w2, int j,
-                                                float& wsum, int& nadd) {
+                                                Row3D& row, float& wsum, int& nadd) {
+    const float xe = s_xe[j];
+    const uint32_t xe_bits = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(__float_as_int(xe)));
+    if (xe_bits != row.xe_bits) {                    // uniform: a new x column
+        row.xe_bits = xe_bits;
+        const float ax = xe - p.px;
+        row.ax2 = ax * ax;
+        if (!FAR) row.ix = window_index(s_dx[j] * p.invx);
+    }
     const uint32_t off = s_off[j];
     bool member = !MAP || off != kNotConnected;
     float w;
@@ -577,9 +596,8 @@ __device__ inline Gather3D<RFT> element3d_issue(__amdgpu_buffer_rsrc_t rf, uint3
     } else {
         member = member && (s_r2[j] <= p.aDT);
         if (member) {
-            const int ix = window_index(s_dx[j] * p.invx);
             const int iz = window_index(s_dz[j] * p.invz);
-            w = s_w2[w2_index(ix, iz)];
+            w = s_w2[w2_index(row.ix, iz)];
         } else {
             w = 0.0f;
         }
@@ -587,8 +605,8 @@ __device__ inline Gather3D<RFT> element3d_issue(__amdgpu_buffer_rsrc_t rf, uint3
     // a skipped element adds +0 to wsum (never -0: window weights are >= 0) and 0 to n
     wsum += w;
     nadd += member ? 1 : 0;
-    const float ax = s_xe[j] - p.px, az = s_ze[j] - p.pz;
-    float tau = t0 + (sqrt_rn((ax * ax + az * az) + p.py2) + p.d);
+    const float az = s_ze[j] - p.pz;
+    float tau = t0 + (sqrt_rn((row.ax2 + az * az) + p.py2) + p.d);
     tau = tau + offset;
     return gather3d_issue<RFT>(rf, off, n_s, tau, w, member);
 }
@@ -598,19 +616,21 @@ __device__ inline void elements3d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float
                                   const float* s_xe, const float* s_ze, const float* s_dx, const float* s_dz,
                                   const float* s_r2, const float* s_w, const uint32_t* s_off, const float* s_w2,
                                   int cnt, float& sample, float& wsum, int& nadd) {
+    // xe is never the NaN pattern 0xffffffff, so the first element starts a row
+    Row3D row{0xffffffffu, 0.0f, 0};
     int j = 0;
     for (; j + kBatch3D <= cnt; j += kBatch3D) {
         Gather3D<RFT> g[kBatch3D];
 #pragma unroll
         for (int u = 0; u < kBatch3D; u++)
             g[u] = element3d_issue<RFT, FAR, MAP>(rf, n_s, t0, offset, p, s_xe, s_ze, s_dx, s_dz, s_r2, s_w, s_off,
-                                                  s_w2, j + u, wsum, nadd);
+                                                  s_w2, j + u, row, wsum, nadd);
 #pragma unroll
         for (int u = 0; u < kBatch3D; u++) gather3d_consume<RFT>(g[u], sample);
     }
     for (; j < cnt; j++) {
         const Gather3D<RFT> g = element3d_issue<RFT, FAR, MAP>(rf, n_s, t0, offset, p, s_xe, s_ze, s_dx, s_dz, s_r2,
-                                                               s_w, s_off, s_w2, j, wsum, nadd);
+                                                               s_w, s_off, s_w2, j, row, wsum, nadd);
         gather3d_consume<RFT>(g, sample);
     }
 }
