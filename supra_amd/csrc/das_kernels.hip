@@ -61,17 +61,15 @@ constexpr int kWavesPerGroup = kThreads / 64;
 #define SUPRA_FRAMES_PER_WAVE 8
 #endif
 constexpr int kFramesPerWave = SUPRA_FRAMES_PER_WAVE;
-// Aperture elements staged per LDS pass (3D).  Small on purpose: the 3D
-// kernel is bound by gather latency, and a small LDS footprint per
-// workgroup (about 12 KB with the packed window table) lets 10 workgroups
-// share a CU.
+// Aperture elements staged per LDS pass (3D).  Small on purpose: a small
+// LDS footprint per workgroup (20 KB with the window table) lets 8
+// workgroups, the 8 waves per SIMD that the registers allow, share a CU.
 constexpr int kChunk3D = 128;
-// sqrt(LUT[i] * LUT[j]) is symmetric in (i, j): upper triangle, row-major
-constexpr int kW2Entries = kWindowEntries * (kWindowEntries + 1) / 2;
-__device__ inline int w2_index(int i, int j) {
-    const int lo = min(i, j), hi = max(i, j);
-    return lo * kWindowEntries - (lo * (lo - 1)) / 2 + (hi - lo);
-}
+// The 3D window weight sqrt(LUT[ix] * LUT[iz]) as a full 64 x 64 table,
+// entry ix * 64 + iz: one add per element with ix * 64 kept per element row
+// (16 KB; with the element chunk a 3D workgroup takes 20 KB of LDS, and 8 of
+// them, 8 waves per SIMD, still fit a CU).
+constexpr int kW2Entries = kWindowEntries * kWindowEntries;
 constexpr uint32_t kNotConnected = 0xffffffffu;
 
 struct Pixel2D {
@@ -571,7 +569,7 @@ constexpr int kBatch3D = 4;   // 2: 5.63 ms, 4: 4.9-5.0 ms, 8: 5.09 ms, 16: 5.27
 struct Row3D {
     uint32_t xe_bits;
     float ax2;
-    int ix;
+    int ix64;    // x window index * 64 (near field)
 };
 
 template <typename RFT, bool FAR, bool MAP>
@@ -586,7 +584,7 @@ __device__ inline Gather3D<RFT> element3d_issue(__amdgpu_buffer_rsrc_t rf, uint3
         row.xe_bits = xe_bits;
         const float ax = xe - p.px;
         row.ax2 = ax * ax;
-        if (!FAR) row.ix = window_index(s_dx[j] * p.invx);
+        if (!FAR) row.ix64 = window_index(s_dx[j] * p.invx) * kWindowEntries;
     }
     const uint32_t off = s_off[j];
     bool member = !MAP || off != kNotConnected;
@@ -597,7 +595,7 @@ __device__ inline Gather3D<RFT> element3d_issue(__amdgpu_buffer_rsrc_t rf, uint3
         member = member && (s_r2[j] <= p.aDT);
         if (member) {
             const int iz = window_index(s_dz[j] * p.invz);
-            w = s_w2[w2_index(row.ix, iz)];
+            w = s_w2[row.ix64 + iz];
         } else {
             w = 0.0f;
         }
@@ -654,7 +652,7 @@ __device__ inline float test_signal_3d(const ScanView& sc, float d, float dt, fl
 
 template <typename RFT, typename OutT, bool TESTSIG, bool INTERP_TX, bool MAP>
 __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
-    __shared__ float s_w2[kW2Entries];   // sqrt(LUT[i] * LUT[j]), packed upper triangle
+    __shared__ float s_w2[kW2Entries];   // sqrt(LUT[ix] * LUT[iz]) at ix * 64 + iz
     __shared__ float s_xe[kChunk3D];
     __shared__ float s_ze[kChunk3D];
     __shared__ float s_dx[kChunk3D];
@@ -668,7 +666,7 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
     // the product and the square root are the same IEEE operations.
     for (int i = threadIdx.x; i < kWindowEntries * kWindowEntries; i += kThreads) {
         const int ix = i / kWindowEntries, iz = i % kWindowEntries;
-        if (iz >= ix) s_w2[w2_index(ix, iz)] = sqrtf(a.lut[ix] * a.lut[iz]);
+        s_w2[i] = sqrtf(a.lut[ix] * a.lut[iz]);
     }
 
     const ScanView sc = scan_view(a.scans, t.s);
@@ -739,7 +737,7 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
                     s_dx[j] = dx;
                     s_dz[j] = dz;
                     s_r2[j] = dx * dx + dz * dz;
-                    s_w[j] = s_w2[w2_index(window_index(dx * invx_far), window_index(dz * invz_far))];
+                    s_w[j] = s_w2[window_index(dx * invx_far) * kWindowEntries + window_index(dz * invz_far)];
                     s_off[j] = (ch == kNotConnected) ? kNotConnected : ch * a.n_s;
                 }
                 __syncthreads();
