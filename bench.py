@@ -145,6 +145,16 @@ def bench_2d(args, torch, frames: int = 32):
     per_frame_ms = ev_ms / frames
     samples = g.num_rx * g.num_depths
     gbs = das_bytes(w, frames) / (ev_ms * 1e-3) / 1e9
+    # one frame per call, as BeamformingNode calls its engine
+    raw1 = USRawData(rf[0], w.fs_hz, g)
+    out1 = out[0]
+
+    def step1():
+        bf.perform_rx_beamforming(0, raw1, kw["f_number"], kw["speed_of_sound_mm_per_s"], kw["window_type"],
+                                  kw["window_parameter"], kw["interpolate_transmits"], kw["additional_offset"],
+                                  out=out1)
+
+    _, ev1_ms = time_launches(step1, max(5, args.steps), 2, torch)
     bf.close()
     res = {
         "workload": f"{w.name}: 128x2080 image, int16 RF 128x64x2080, {frames} frames per launch",
@@ -152,6 +162,8 @@ def bench_2d(args, torch, frames: int = 32):
         "us_per_frame": per_frame_ms * 1e3,
         "achieved_GBs": gbs,
         "hbm_frac": gbs / HBM_PEAK_GBS,
+        "single_frame_call_us": ev1_ms * 1e3,
+        "single_frame_hbm_frac": das_bytes(w, 1) / (ev1_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_2d(w, g, rf, out, kw, args.cpu_budget / 3)

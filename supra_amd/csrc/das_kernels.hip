@@ -76,37 +76,107 @@ struct Pixel2D {
     float d, aDT, inv, px, py2;
 };
 
+// One 2D element's gathers, issued before any of a batch is consumed
+// (gather<RFT, false>'s loads and decisions, RxSampleBeamformerDelayAndSum.h:
+// 174-187).  A skipped element (outside the aperture, unconnected channel)
+// has member == false and weight 0.
+template <typename RFT>
+struct Gather2D {
+    float v0, v1, f, w;
+    bool add;    // int16: member and 0 <= tau <= N-1; float: member and (inr | edge)
+    bool edge;   // float: the last sample with zero fraction
+};
+
+template <typename RFT>
+__device__ inline Gather2D<RFT> gather2d_issue(__amdgpu_buffer_rsrc_t rf, uint32_t off, uint32_t n_s, float tau,
+                                               float w, bool member) {
+    Gather2D<RFT> g;
+    const uint32_t ui = static_cast<uint32_t>(static_cast<int32_t>(floorf(tau)));
+    g.f = __builtin_amdgcn_fractf(tau);
+    g.w = w;
+    g.v0 = load_rf<RFT>(rf, off + ui);
+    g.v1 = load_rf<RFT>(rf, off + ui + 1u);
+    if constexpr (std::is_same_v<RFT, int16_t>) {
+        const float last = static_cast<float>(n_s - 1u);
+        g.add = member & (tau >= 0.0f) & (tau <= last);
+        g.edge = false;
+    } else {
+        const bool inr = ui < n_s - 1u;
+        g.edge = (ui == n_s - 1u) & (g.f == 0.0f);
+        g.add = member & (inr | g.edge);
+    }
+    return g;
+}
+
+// The element's contribution in gather()'s operations; a skipped element adds
+// +0 (int16: the sum is never -0) or leaves the sum selected unchanged.
+template <typename RFT>
+__device__ inline void gather2d_consume(const Gather2D<RFT>& g, float& sample) {
+    if constexpr (std::is_same_v<RFT, int16_t>) {
+        const float contrib = g.w * ((1.0f - g.f) * g.v0 + g.f * g.v1);
+        sample += g.add ? contrib : 0.0f;
+    } else {
+        const float val = g.edge ? g.v0 : ((1.0f - g.f) * g.v0 + g.f * g.v1);
+        const float contrib = g.w * val;
+        sample = g.add ? sample + contrib : sample;
+    }
+}
+
 // Elements [0, cnt) of the current LDS chunk for one 2D pixel
-// (RxSampleBeamformerDelayAndSum.h:150-199).
+// (RxSampleBeamformerDelayAndSum.h:150-199), in batches of kBatch2D whose
+// delays and gathers all go out before the first is consumed (as in 3D);
+// weights, counts and samples are added in element order.
+constexpr int kBatch2D = 4;
+
+template <typename RFT, bool FAR, bool VERT, bool MAP>
+__device__ inline Gather2D<RFT> element2d_issue(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float t0, float offset,
+                                                const Pixel2D& p, const float* s_xe, const float* s_dx,
+                                                const float* s_dx2, const float* s_w, const uint32_t* s_off,
+                                                const float* s_lut, int j, float& wsum, int& nadd) {
+    const uint32_t off = s_off[j];
+    bool member = !MAP || off != kNotConnected;
+    float w;
+    if (FAR) {
+        w = member ? s_w[j] : 0.0f;
+    } else {
+        const float dx = s_dx[j];
+        member = member && (fabsf(dx) <= p.aDT);
+        w = member ? s_lut[window_index(dx * p.inv)] : 0.0f;
+    }
+    // a skipped element adds +0 to wsum (never -0: window weights are >= 0) and 0 to n
+    wsum += w;
+    nadd += member ? 1 : 0;
+    float q;
+    if (VERT) {
+        q = s_dx2[j] + p.py2;
+    } else {
+        const float e = s_xe[j] - p.px;
+        q = e * e + p.py2;
+    }
+    float tau = t0 + (sqrt_rn(q) + p.d);
+    tau = tau + offset;
+    return gather2d_issue<RFT>(rf, off, n_s, tau, w, member);
+}
+
 template <typename RFT, bool FAR, bool VERT, bool MAP>
 __device__ inline void elements2d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float t0, float offset, const Pixel2D& p,
                                   const float* s_xe, const float* s_dx, const float* s_dx2, const float* s_w,
                                   const uint32_t* s_off, const float* s_lut, int cnt, float& sample, float& wsum,
                                   int& nadd) {
-#pragma unroll 4
-    for (int j = 0; j < cnt; j++) {
-        const uint32_t off = s_off[j];
-        if (MAP && off == kNotConnected) continue;
-        float w;
-        if (FAR) {
-            w = s_w[j];
-        } else {
-            const float dx = s_dx[j];
-            if (!(fabsf(dx) <= p.aDT)) continue;
-            w = s_lut[window_index(dx * p.inv)];
-        }
-        wsum += w;
-        nadd++;
-        float q;
-        if (VERT) {
-            q = s_dx2[j] + p.py2;
-        } else {
-            const float e = s_xe[j] - p.px;
-            q = e * e + p.py2;
-        }
-        float tau = t0 + (sqrt_rn(q) + p.d);
-        tau = tau + offset;
-        gather<RFT, false>(rf, off, n_s, tau, w, sample);
+    int j = 0;
+    for (; j + kBatch2D <= cnt; j += kBatch2D) {
+        Gather2D<RFT> g[kBatch2D];
+#pragma unroll
+        for (int u = 0; u < kBatch2D; u++)
+            g[u] = element2d_issue<RFT, FAR, VERT, MAP>(rf, n_s, t0, offset, p, s_xe, s_dx, s_dx2, s_w, s_off, s_lut,
+                                                        j + u, wsum, nadd);
+#pragma unroll
+        for (int u = 0; u < kBatch2D; u++) gather2d_consume<RFT>(g[u], sample);
+    }
+    for (; j < cnt; j++) {
+        const Gather2D<RFT> g = element2d_issue<RFT, FAR, VERT, MAP>(rf, n_s, t0, offset, p, s_xe, s_dx, s_dx2, s_w,
+                                                                     s_off, s_lut, j, wsum, nadd);
+        gather2d_consume<RFT>(g, sample);
     }
 }
 
