@@ -688,6 +688,15 @@ __device__ inline void elements3d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float
     Row3D row{0xffffffffu, 0.0f, 0};
     int j = 0;
     for (; j + kBatch3D <= cnt; j += kBatch3D) {
+        if (!FAR) {
+            // near field: a batch that no lane of the wave has in its
+            // aperture adds nothing (+0 weights, no samples) and is skipped
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < kBatch3D; u++)
+                any |= (s_r2[j + u] <= p.aDT) && (!MAP || s_off[j + u] != kNotConnected);
+            if (__builtin_amdgcn_ballot_w64(any) == 0) continue;   // uniform
+        }
         Gather3D<RFT> g[kBatch3D];
 #pragma unroll
         for (int u = 0; u < kBatch3D; u++)
