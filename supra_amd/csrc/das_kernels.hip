@@ -165,6 +165,15 @@ __device__ inline void elements2d(__amdgpu_buffer_rsrc_t rf, uint32_t n_s, float
                                   int& nadd) {
     int j = 0;
     for (; j + kBatch2D <= cnt; j += kBatch2D) {
+        if (!FAR) {
+            // near field: a batch that no lane of the wave has in its
+            // aperture adds nothing (+0 weights, no samples) and is skipped
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < kBatch2D; u++)
+                any |= (fabsf(s_dx[j + u]) <= p.aDT) && (!MAP || s_off[j + u] != kNotConnected);
+            if (__builtin_amdgcn_ballot_w64(any) == 0) continue;   // uniform
+        }
         Gather2D<RFT> g[kBatch2D];
 #pragma unroll
         for (int u = 0; u < kBatch2D; u++)
