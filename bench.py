@@ -166,31 +166,35 @@ def bench_2d(args, torch, frames: int = 32):
         "single_frame_hbm_frac": das_bytes(w, 1) / (ev1_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_2d(w, g, rf, out, kw, args.cpu_budget / 3)
+        res["cpu_baseline"] = cpu_baseline_2d(w, g, rf, out, kw, args.cpu_budget)
         res["gpu_over_cpu"] = res["samples_per_s"] / res["cpu_baseline"]["value"]
     return res
 
 
 def cpu_baseline_2d(w, g, rf, out, kw, budget_s: float):
-    """Host FP32 oracle on whole frames of the same 2D batch (one frame after
-    another until budget_s has passed), each checked against the GPU image."""
+    """Host FP32 oracle on whole frames of the same 2D batch, one frame after
+    another (cycling through the batch) until budget_s has passed; each
+    frame's first pass is checked against the GPU image."""
     from oracle import oracle as O
 
     cpu = host_cpu()
     threads = cpu["threads"]
     rf_host = rf.cpu().numpy()
     out_host = out.cpu().numpy()
+    n = rf_host.shape[0]
     done, exact, elapsed = 0, True, 0.0
-    while done < rf_host.shape[0] and (done == 0 or elapsed < budget_s):
+    while done == 0 or elapsed < budget_s:
+        f = done % n
         t0 = time.perf_counter()
-        img = O.das(g, rf_host[done], w.fs_hz, threads=threads, **kw)
+        img = O.das(g, rf_host[f], w.fs_hz, threads=threads, **kw)
         elapsed += time.perf_counter() - t0
-        exact = exact and bool(np.array_equal(img, out_host[done]))
+        if done < n:
+            exact = exact and bool(np.array_equal(img, out_host[f]))
         done += 1
     ref64 = O.das(g, rf_host[0], w.fs_hz, precision=64, threads=threads, **kw)
     return {"value": done * g.num_rx * g.num_depths / elapsed, "unit": "samples/s", "cores": threads,
-            "kind": "port", "sample": f"{done} whole frames ({done * g.num_rx * g.num_depths} samples, "
-                                      f"{elapsed:.1f} s)",
+            "kind": "port", "sample": f"{done} whole frames, cycling through the {n}-frame batch "
+                                      f"({done * g.num_rx * g.num_depths} samples, {elapsed:.1f} s)",
             "host": cpu, "gpu_matches_sample_bit_exact": exact,
             "gpu_normwise_err_vs_fp64": normwise(out_host[0], ref64)}
 
