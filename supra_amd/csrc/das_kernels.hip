@@ -207,9 +207,8 @@ struct Tile {
     bool valid;
 };
 
-__device__ inline Tile decode_tile(const DasArgs& a) {
+__device__ inline Tile decode_tile(const DasArgs& a, uint32_t bid = blockIdx.x) {
     Tile t;
-    const uint32_t bid = blockIdx.x;
     const uint32_t tile = bid % a.depth_tiles;
     const uint32_t rest = bid / a.depth_tiles;
     t.s = rest % a.n_rx;
@@ -762,7 +761,13 @@ __global__ __launch_bounds__(kThreads) void das3d_kernel(const DasArgs a) {
     __shared__ float s_w[kChunk3D];
     __shared__ uint32_t s_off[kChunk3D];
 
-    const Tile t = decode_tile(a);
+    // All depth tiles of a scanline form one XCD tile (logical_wg_2d): the
+    // RF lines two depth tiles share at their boundary are fetched into one
+    // L2, and the XCDs take whole scanlines.  The grid is padded to whole
+    // rounds; a padded workgroup has no tile.
+    const uint32_t lb = logical_wg_2d(a.depth_tiles);
+    if (lb >= a.depth_tiles * a.n_rx * a.n_frames) return;   // whole workgroup
+    const Tile t = decode_tile(a, lb);
     // computeWindow3DShared (RxBeamformerCommon.h:54-61) as a 64x64 table:
     // the product and the square root are the same IEEE operations.
     for (int i = threadIdx.x; i < kWindowEntries * kWindowEntries; i += kThreads) {
@@ -1122,7 +1127,8 @@ template <typename RFT, typename OutT>
 static hipError_t launch_typed(const DasArgs& a, bool is3d, bool testsig, bool interp_tx, bool map,
                                hipStream_t stream) {
     const uint64_t units2d = static_cast<uint64_t>((a.n_depths + kWaveDepths - 1) / kWaveDepths) * a.n_rx * a.n_frames;
-    const uint64_t blocks = is3d ? static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames
+    const uint64_t round3d = static_cast<uint64_t>(kXcds) * a.depth_tiles;   // whole rounds of scanline tiles
+    const uint64_t blocks = is3d ? (static_cast<uint64_t>(a.depth_tiles) * a.n_rx * a.n_frames + round3d - 1) / round3d * round3d
                                  : pad_2d_grid((units2d + kWavesPerGroup - 1) / kWavesPerGroup);
     dim3 grid(static_cast<uint32_t>(blocks)), block(kThreads);
 #define SUPRA_DAS_DISPATCH(KERNEL)                                                                                 \
