@@ -216,6 +216,26 @@ def test_ragged_sizes_bit_exact(device, num_rx, num_depths, aperture):
     assert_bit_exact(gpu, oracle_das(g, rf, w.fs_hz, **kw), f"{num_rx}x{num_depths} aperture {aperture}")
 
 
+@pytest.mark.parametrize("num_rx,num_depths,n_frames", [(40, 130, 3), (7, 65, 2), (33, 257, 9), (64, 2, 2)])
+def test_frames_ragged_sizes_bit_exact(device, num_rx, num_depths, n_frames):
+    """Several frames per launch (the multi-frame kernel, tiled by 32
+    scanlines x 2 depth chunks) at sizes that leave partial tiles: scanline
+    counts off the 32 grid, odd and even chunk counts, a partial frame group.
+    Every frame matches the single-frame oracle bit for bit."""
+    depth_mm = num_depths / 40e6 * 1540e3 / 2.0
+    w = W.linear_acquisition(num_rx, num_depths, 16, 40e6, 20, scatter_seed=9, noise_seed=10,
+                             box_depth=(0.0, depth_mm))
+    g = w.geometry()
+    kw = w.das_kwargs()
+    frames = [W.linear_acquisition(num_rx, num_depths, 16, 40e6, 20, scatter_seed=9 + f, noise_seed=10 + f,
+                                   box_depth=(0.0, depth_mm)).rf_host() for f in range(n_frames)]
+    batch = gpu_das(g, np.stack(frames), w.fs_hz, **kw)
+    assert batch.shape == (n_frames, num_depths, num_rx)
+    for f in range(n_frames):
+        assert_bit_exact(batch[f], oracle_das(g, frames[f], w.fs_hz, **kw),
+                         f"frame {f} of {n_frames}, {num_rx}x{num_depths}")
+
+
 @pytest.mark.parametrize("n_depths", [2, 65, 130])
 def test_matrix_3d_ragged_depths_bit_exact(device, n_depths):
     w = W.matrix_3d_small(n_el=4, n_lines=4, n_depths=n_depths, n_samples=600)
