@@ -687,15 +687,30 @@ __device__ __attribute__((always_inline)) inline void mv_tile(const MvArgs& a, c
     }
 }
 
+// Workgroup b runs on XCD b % 8 (round robin).  Consecutive depth tiles of a
+// scanline share their +-T halo of staged samples; renumbered in XCD tiles of
+// kMvXcdTile consecutive workgroups (rounds of 8 tiles rotated by one XCD per
+// round) they run on one XCD and re-read the halo from its L2.  A bijection
+// on the grid, which the launch pads to whole rounds.
+constexpr uint32_t kMvXcdTile = 32;   // C3: 4 / 8 / 16 / 32 / 64 / 128 -> 2.045 / 2.04 / 2.035 / 2.025 / 2.02 / 2.06 ms
+__device__ inline uint32_t mv_logical_wg() {
+    const uint32_t b = blockIdx.x, xcd = b % 8u, local = b / 8u;
+    const uint32_t round = local / kMvXcdTile, k = local % kMvXcdTile;
+    return (round * 8u + (xcd + round) % 8u) * kMvXcdTile + k;
+}
+
 template <typename InT, typename OutT, int LMAX, bool MEANS, bool FUSED>
 __global__ __launch_bounds__(kThreads) void mv_kernel(const MvArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
-    mv_tile<InT, OutT, LMAX, MEANS, FUSED>(a, a.g.lut, blockIdx.x / a.n_tiles, blockIdx.x % a.n_tiles, lds);
+    const uint32_t b = mv_logical_wg();
+    if (b >= a.n_rx * a.n_tiles) return;   // whole workgroup: the padded grid
+    mv_tile<InT, OutT, LMAX, MEANS, FUSED>(a, a.g.lut, b / a.n_tiles, b % a.n_tiles, lds);
 }
 
 template <typename InT, typename OutT, int LMAX, bool FUSED>
 hipError_t launch_typed(const MvArgs& a, size_t lds, hipStream_t stream) {
-    dim3 grid(a.n_rx * a.n_tiles), block(kThreads);
+    const uint32_t round = 8u * kMvXcdTile;
+    dim3 grid((a.n_rx * a.n_tiles + round - 1) / round * round), block(kThreads);
     if (a.means) hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, true, FUSED>), grid, block, lds, stream, a);
     else hipLaunchKernelGGL((mv_kernel<InT, OutT, LMAX, false, FUSED>), grid, block, lds, stream, a);
     return hipGetLastError();
