@@ -857,10 +857,13 @@ extern "C" int supra_das_adjoint(supra_bf_ctx* ctx, uint32_t num_frames, const f
     const double dt = 1.0 / sampling_frequency_hz;
     FrameTables ft;
     if ((st = build_frame_tables(ctx, num_frames, speed_of_sound_mm_per_s, dt, hs, &ft)) != SUPRA_OK) return st;
+    // beta, then the per-frame depth table (4 floats per depth), in one buffer
     const size_t n_beta = static_cast<size_t>(num_frames) * slots * ctx->n_rx * ctx->n_depths;
-    if ((st = ensure_capacity(&ctx->d_beta, &ctx->beta_cap, n_beta * sizeof(float), "das adjoint: scratch")) !=
-        SUPRA_OK)
+    const size_t b_beta = align256(n_beta * sizeof(float));
+    const size_t b_dtab = static_cast<size_t>(num_frames) * ctx->n_depths * 4 * sizeof(float);
+    if ((st = ensure_capacity(&ctx->d_beta, &ctx->beta_cap, b_beta + b_dtab, "das adjoint: scratch")) != SUPRA_OK)
         return st;
+    float* dtab = reinterpret_cast<float*>(static_cast<char*>(ctx->d_beta) + b_beta);
     const size_t per_frame = static_cast<size_t>(ctx->n_rx) * slots * egroups * waves;
     if (grad_c && (st = ensure_capacity(&ctx->d_dc_part, &ctx->dc_part_cap, per_frame * num_frames * sizeof(double),
                                         "das adjoint: scratch")) != SUPRA_OK)
@@ -876,13 +879,16 @@ extern "C" int supra_das_adjoint(supra_bf_ctx* ctx, uint32_t num_frames, const f
     e = launch_beta(grad_image, norm, static_cast<float*>(ctx->d_beta), ctx->n_rx, ctx->n_depths, slots, num_frames,
                     hs);
     if (e != hipSuccess) return hip_fail(e, "das adjoint: beta launch");
+    e = launch_depth_table(ft.depths, ft.c, static_cast<float>(dt), static_cast<float>(f_number), ctx->n_depths,
+                           num_frames, dtab, hs);
+    if (e != hipSuccess) return hip_fail(e, "das adjoint: depth table launch");
 
     AdjointArgs a;
     std::memset(&a, 0, sizeof(a));
     a.rf = rf;
     a.beta = static_cast<const float*>(ctx->d_beta);
     a.scans = ft.scans;
-    a.depths = ft.depths;
+    a.dtab = dtab;
     a.ex = ft.ex;
     a.frame_c = ft.c;
     a.e2c = ctx->d_e2c;

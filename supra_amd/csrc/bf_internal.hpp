@@ -101,7 +101,7 @@ struct AdjointArgs {
     const float* rf;                // [frame][tx][ch][s]
     const float* beta;              // [frame][slot][scanline][depth]: dL/dy * norm
     const ScanlineDT* scans;        // per frame (scan_frame_stride)
-    const float* depths;            // per frame (depth_frame_stride)
+    const float* dtab;              // per frame (depth_frame_stride): 4 floats {depth, aDT, 1/aDT, 0} per depth
     const float* ex;                // per frame (el_frame_stride)
     const float* frame_c;           // [frame] float c
     const int32_t* e2c;             // or nullptr

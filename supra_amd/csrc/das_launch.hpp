@@ -32,6 +32,8 @@ size_t das_batch_frame_bytes();
 
 // Differentiable 2D DAS (das_adjoint.hip).
 hipError_t launch_frame_tables(const FrameTablesArgs& a, hipStream_t stream);
+hipError_t launch_depth_table(const float* depths, const float* frame_c, float dt, float f_number, uint32_t n_d,
+                              uint32_t frames, float* dtab, hipStream_t stream);
 hipError_t launch_beta(const float* grad_image, const float* norm, float* beta, uint32_t n_rx, uint32_t n_d,
                        uint32_t slots, uint32_t frames, hipStream_t stream);
 size_t das_adjoint_lds_bytes(uint32_t waves, uint32_t n_s);

@@ -91,6 +91,12 @@ CASES = [
     ("hamming_offset", dict(window=N.WINDOW_HAMMING, additional_offset=5), dict()),
     ("subdivided_interp", dict(interpolate_transmits=True),
      dict(num_scanlines=(64, 1), rx_scanline_subdivision=(2, 1))),
+    # 96 depths over 40 mm: 11-22 samples per depth, so the delays of one
+    # 64-depth chunk span more than the kernel's 256-sample ring (ring moved
+    # to the chunk's largest target, the rest added to the flushed line).
+    ("coarse_depths", dict(), dict(num_depths=96)),
+    ("coarse_depths_interp", dict(interpolate_transmits=True),
+     dict(num_depths=96, num_scanlines=(64, 1), rx_scanline_subdivision=(2, 1))),
 ]
 
 
