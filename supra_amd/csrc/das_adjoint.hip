@@ -275,6 +275,10 @@ __global__ __launch_bounds__(256) void das2d_adjoint_kernel(const AdjointArgs a)
             }
             // Same single-precision operations as the forward kernel.
             const bool take = valid && bt != 0.0f && fabsf(dx) <= aDT;
+            // No lane of this chunk in the element's aperture (shallow depths
+            // of the outer elements): nothing to add; the ring catches up at
+            // the next chunk that has work.
+            if (__ballot(take) == 0ull) continue;
             const float w = s_lut[window_index(dx * inv)];
             const float px = x + dir_x * d;
             const float py = dir_y * d;
