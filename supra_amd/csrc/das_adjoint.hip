@@ -56,6 +56,12 @@ namespace supra_amd {
 namespace {
 
 constexpr int kTableThreads = 256;
+constexpr uint32_t kThrParts = 4;                // element strides per transmit slot
+constexpr uint32_t kScanLanes = 4 * kThrParts;   // lanes per scanline in frame_tables_kernel
+
+__host__ __device__ inline uint32_t scan_items_start(uint32_t n_d, uint32_t n_el) {
+    return (n_d + n_el + 63u) / 64u * 64u;
+}
 constexpr int kBetaTile = 64;
 constexpr int kWave = 64;
 // Each adjoint wave accumulates its RF line in a ring of kRing samples of LDS;
@@ -85,24 +91,44 @@ __global__ __launch_bounds__(kTableThreads) void frame_tables_kernel(const Frame
         a.ex[static_cast<size_t>(f) * a.n_el + j] = static_cast<float>(a.ex_mm[j] * factor);
         return;
     }
-    j -= a.n_el;
-    if (j >= a.n_rx) return;
+    // Scanlines: kScanLanes consecutive lanes per scanline (4 transmit slots x
+    // kThrParts element strides), from the first item at a multiple of the
+    // wave size, so that each group is one aligned shuffle segment; the
+    // far-field threshold of a slot is the largest |ex - x| over its
+    // connected elements, a max, so the parts reduce in any order.
+    const uint32_t s0 = scan_items_start(a.n_d, a.n_el);
+    if (i < s0) return;
+    j = i - s0;
+    const uint32_t sub = j % kScanLanes;
+    j /= kScanLanes;
+    if (j >= a.n_rx) return;   // whole groups of kScanLanes lanes
     const supra_scanline_rx& s = a.scan_mm[j];
     ScanlineDT o = a.scan_fixed[j];
     o.x = static_cast<float>(s.position[0] * factor);
     o.z = static_cast<float>(s.position[2] * factor);
     o.max_ed_x = static_cast<float>(s.max_element_distance[0] * factor);
     o.max_ed_y = static_cast<float>(s.max_element_distance[1] * factor);
-    for (int k = 0; k < 4; k++) {
-        TxDT& t = o.tx[k];
-        t.t0 = static_cast<float>(s.tx[k].initial_delay * factor_time);
-        float thr = 0.0f;
-        for (uint32_t e = t.first_x; e < t.last_x; e++) {
+    const uint32_t kk = sub / kThrParts, part = sub % kThrParts;
+    float thr = 0.0f;
+    {
+        const TxDT& t = o.tx[kk];
+        for (uint32_t e = t.first_x + part; e < t.last_x; e += kThrParts) {
             if (a.e2c && a.e2c[e] < 0) continue;
             const float v = fabsf(static_cast<float>(a.ex_mm[e] * factor) - o.x);
             thr = (thr < v) ? v : thr;
         }
-        t.far_thr = (thr < o.max_ed_x) ? o.max_ed_x : thr;
+    }
+    for (uint32_t m = 1; m < kThrParts; m <<= 1) {
+        const float v = __shfl_xor(thr, static_cast<int>(m), kScanLanes);
+        thr = (thr < v) ? v : thr;
+    }
+    float thr_k[4];
+    for (int k = 0; k < 4; k++) thr_k[k] = __shfl(thr, k * static_cast<int>(kThrParts), kScanLanes);
+    if (sub != 0) return;
+    for (int k = 0; k < 4; k++) {
+        TxDT& t = o.tx[k];
+        t.t0 = static_cast<float>(s.tx[k].initial_delay * factor_time);
+        t.far_thr = (thr_k[k] < o.max_ed_x) ? o.max_ed_x : thr_k[k];
     }
     a.scans[static_cast<size_t>(f) * a.n_rx + j] = o;
 }
@@ -380,15 +406,18 @@ __global__ __launch_bounds__(256) void das2d_adjoint_kernel(const AdjointArgs a)
 }
 
 // dL/dc per frame: fixed-order sum of the waves' partial sums, times -1/c.
-__global__ __launch_bounds__(256) void dc_reduce_kernel(const double* part, uint32_t per_frame, const double* c,
-                                                        double* grad_c) {
-    __shared__ double red[256];
+constexpr uint32_t kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void dc_reduce_kernel(const double* part, uint32_t per_frame,
+                                                                   const double* c, double* grad_c) {
+    __shared__ double red[kReduceThreads];
     const uint32_t f = blockIdx.x;
     double acc = 0.0;
-    for (uint32_t i = threadIdx.x; i < per_frame; i += 256) acc += part[static_cast<size_t>(f) * per_frame + i];
+    for (uint32_t i = threadIdx.x; i < per_frame; i += kReduceThreads)
+        acc += part[static_cast<size_t>(f) * per_frame + i];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (uint32_t w = 128; w > 0; w >>= 1) {
+    for (uint32_t w = kReduceThreads / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -411,7 +440,7 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(float* p, uint64_t n) {
 }  // namespace
 
 hipError_t launch_frame_tables(const FrameTablesArgs& a, hipStream_t stream) {
-    const uint32_t items = a.n_d + a.n_el + a.n_rx;
+    const uint32_t items = scan_items_start(a.n_d, a.n_el) + a.n_rx * kScanLanes;
     dim3 grid((items + kTableThreads - 1) / kTableThreads, a.n_frames), block(kTableThreads);
     hipLaunchKernelGGL(frame_tables_kernel, grid, block, 0, stream, a);
     return hipGetLastError();
@@ -444,7 +473,7 @@ hipError_t launch_das_adjoint(const AdjointArgs& a, uint32_t frames, hipStream_t
 
 hipError_t launch_dc_reduce(const double* part, uint32_t per_frame, const double* c, double* grad_c, uint32_t frames,
                             hipStream_t stream) {
-    hipLaunchKernelGGL(dc_reduce_kernel, dim3(frames), dim3(256), 0, stream, part, per_frame, c, grad_c);
+    hipLaunchKernelGGL(dc_reduce_kernel, dim3(frames), dim3(kReduceThreads), 0, stream, part, per_frame, c, grad_c);
     return hipGetLastError();
 }
 
