@@ -132,6 +132,13 @@ def test_convert_bit_exact_on_library_tables(device, name, in_dt, out_dt):
     same = (got == want) | (np.isnan(got) & np.isnan(want))
     assert same.all(), f"{(~same).sum()} of {got.size} pixels differ"
     assert np.array_equal(single.cpu().numpy().ravel(), got[1], equal_nan=True)
+    # an output one element into a larger buffer is not aligned to the packed
+    # stores of 4 pixels: the per-pixel kernel writes it, with the same values
+    shape = (3,) + tuple(single.shape)
+    buf = torch.zeros(int(np.prod(shape)) + 1, dtype=tt[out_dt], device="cuda")
+    shifted = sc.convert(dev, tt[out_dt], out=buf[1:].view(shape))
+    torch.cuda.synchronize()
+    assert np.array_equal(shifted.cpu().numpy().reshape(3, -1), got, equal_nan=True)
     # outside the scan the image is 0
     assert np.all(got[:, mask == 0] == 0)
 
