@@ -838,14 +838,10 @@ extern "C" int supra_das_adjoint(supra_bf_ctx* ctx, uint32_t num_frames, const f
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     if ((st = check_device(ctx)) != SUPRA_OK) return st;
-    // Waves (aperture elements) per workgroup: each holds one RF line in LDS.
-    int dev_lds = 0;
-    hipError_t e = hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
-    if (e != hipSuccess) return hip_fail(e, "das adjoint: device attribute");
-    uint32_t waves = 4;
-    while (waves > 1 && das_adjoint_lds_bytes(waves, num_samples) > 65536) waves /= 2;
-    if (das_adjoint_lds_bytes(waves, num_samples) > static_cast<size_t>(dev_lds))
-        return fail(SUPRA_ERR_TOO_LARGE, "das adjoint: %u samples per channel do not fit in LDS", num_samples);
+    // Waves (aperture elements) per workgroup: each accumulates its RF line in
+    // a fixed-size LDS ring, so any number of samples per channel fits.
+    hipError_t e = hipSuccess;
+    const uint32_t waves = 4;
     const uint32_t slots = interpolate_transmits ? 4u : 1u;
     const uint32_t egroups = std::max(1u, (ctx->max_aperture + waves - 1) / waves);
     const uint64_t blocks = static_cast<uint64_t>(num_frames) * ctx->n_rx * slots * egroups;

@@ -460,14 +460,14 @@ hipError_t launch_beta(const float* grad_image, const float* norm, float* beta, 
     return hipGetLastError();
 }
 
-size_t das_adjoint_lds_bytes(uint32_t waves, uint32_t /*n_s*/) {
+size_t das_adjoint_lds_bytes(uint32_t waves) {
     return (static_cast<size_t>(kWindowEntries) + static_cast<size_t>(waves) * kRing) * sizeof(float);
 }
 
 hipError_t launch_das_adjoint(const AdjointArgs& a, uint32_t frames, hipStream_t stream) {
     const uint64_t blocks = static_cast<uint64_t>(frames) * a.n_rx * a.n_slots * a.egroups;
     dim3 grid(static_cast<uint32_t>(blocks)), block(a.waves * kWave);
-    hipLaunchKernelGGL(das2d_adjoint_kernel, grid, block, das_adjoint_lds_bytes(a.waves, a.n_s), stream, a);
+    hipLaunchKernelGGL(das2d_adjoint_kernel, grid, block, das_adjoint_lds_bytes(a.waves), stream, a);
     return hipGetLastError();
 }
 

@@ -36,7 +36,7 @@ hipError_t launch_depth_table(const float* depths, const float* frame_c, float d
                               uint32_t frames, float* dtab, hipStream_t stream);
 hipError_t launch_beta(const float* grad_image, const float* norm, float* beta, uint32_t n_rx, uint32_t n_d,
                        uint32_t slots, uint32_t frames, hipStream_t stream);
-size_t das_adjoint_lds_bytes(uint32_t waves, uint32_t n_s);
+size_t das_adjoint_lds_bytes(uint32_t waves);
 hipError_t launch_das_adjoint(const AdjointArgs& a, uint32_t frames, hipStream_t stream);
 hipError_t launch_dc_reduce(const double* part, uint32_t per_frame, const double* c, double* grad_c, uint32_t frames,
                             hipStream_t stream);

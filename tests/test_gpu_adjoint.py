@@ -97,6 +97,9 @@ CASES = [
     ("coarse_depths", dict(), dict(num_depths=96)),
     ("coarse_depths_interp", dict(interpolate_transmits=True),
      dict(num_depths=96, num_scanlines=(64, 1), rx_scanline_subdivision=(2, 1))),
+    # 20,000-sample RF lines: far more than LDS holds per wave (the kernel
+    # keeps a 256-sample ring of each line).
+    ("long_lines", dict(num_samples=20000), dict(num_scanlines=(16, 1), num_depths=300)),
 ]
 
 
