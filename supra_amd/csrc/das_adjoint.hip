@@ -62,7 +62,7 @@ constexpr uint32_t kScanLanes = 4 * kThrParts;   // lanes per scanline in frame_
 __host__ __device__ inline uint32_t scan_items_start(uint32_t n_d, uint32_t n_el) {
     return (n_d + n_el + 63u) / 64u * 64u;
 }
-constexpr int kBetaTile = 64;
+constexpr int kBetaTile = 32;
 constexpr int kWave = 64;
 // Each adjoint wave accumulates its RF line in a ring of kRing samples of LDS;
 // samples more than kRingGuard below the delay of the chunk's first depth are
