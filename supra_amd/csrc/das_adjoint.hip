@@ -286,8 +286,7 @@ __global__ __launch_bounds__(256) void das2d_adjoint_kernel(const AdjointArgs a)
         // Lanes are consecutive depths, so their delays (never decreasing with
         // depth) are sorted across the wave.  A lane whose upper sample ui+1
         // is the next lane's lower sample hands its share to that lane, which
-        // adds it before its own LDS add: one LDS float atomic instead of two
-        // for that sample.
+        // adds it with its own: one add instead of two for that sample.
         for (uint32_t r0 = 0; r0 < a.n_depths; r0 += kWave) {
             const uint32_t r = r0 + lane;
             const bool valid = r < a.n_depths;
