@@ -270,36 +270,147 @@ template <> __device__ inline uint8_t clamp_out<uint8_t>(float x) {
     return static_cast<uint8_t>(fminf(fmaxf(x, 0.0f), 255.0f));
 }
 
-template <typename InT, typename OutT, bool IS3D>
+// V pixels per thread, consecutive in the output: V = 4 stores each frame's
+// 4 pixels with one packed store; the host takes it when the pixel count is a
+// multiple of 4 and the output is aligned to the packed size.
+template <typename OutT, int V> struct alignas(V * sizeof(OutT)) OutPack {
+    OutT v[V];
+};
+
+template <typename InT, typename OutT, bool IS3D, int V>
 __global__ __launch_bounds__(256) void scan_convert_kernel(const ConvArgs a) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.n_pixels) return;
-    const uint32_t s32 = a.idx[i];
-    const uint64_t s = s32, n = a.n_rx, nx = a.n_rx_x;
-    // the last sample row has no successor: the reference's 3D guard, used in 2D too
-    const bool valid = s32 != kOutside && (IS3D ? s + 1 + nx + n : s + 1 + n) < a.in_frame;
-    float wx = 0, wy = 0, wz = 0;
-    if (valid) {
-        wx = a.wx[i];
-        wy = a.wy[i];
-        if (IS3D) wz = a.wz[i];
+    const uint64_t i0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * V;
+    if (i0 >= a.n_pixels) return;
+    const uint64_t n = a.n_rx, nx = a.n_rx_x;
+    uint64_t s[V];
+    bool valid[V];
+    float wx[V], wy[V], wz[V];
+    for (int v = 0; v < V; v++) {
+        const uint64_t i = i0 + v;
+        const uint32_t s32 = a.idx[i];
+        s[v] = s32;
+        // the last sample row has no successor: the reference's 3D guard, used in 2D too
+        valid[v] = s32 != kOutside && (IS3D ? s[v] + 1 + nx + n : s[v] + 1 + n) < a.in_frame;
+        wx[v] = valid[v] ? a.wx[i] : 0.0f;
+        wy[v] = valid[v] ? a.wy[i] : 0.0f;
+        wz[v] = (IS3D && valid[v]) ? a.wz[i] : 0.0f;
     }
     const InT* in = static_cast<const InT*>(a.in);
     OutT* out = static_cast<OutT*>(a.out);
     for (uint32_t f = 0; f < a.num_frames; f++, in += a.in_frame, out += a.n_pixels) {
-        float val = 0.0f;
-        if (valid) {
-            if (!IS3D) {
-                val = (1 - wy) * ((1 - wx) * in_at(in, s) + wx * in_at(in, s + 1)) +
-                      wy * ((1 - wx) * in_at(in, s + n) + wx * in_at(in, s + 1 + n));
-            } else {
-                val = (1 - wy) * ((1 - wz) * ((1 - wx) * in_at(in, s) + wx * in_at(in, s + 1)) +
-                                  wz * ((1 - wx) * in_at(in, s + nx) + wx * in_at(in, s + 1 + nx))) +
-                      wy * ((1 - wz) * ((1 - wx) * in_at(in, s + n) + wx * in_at(in, s + 1 + n)) +
-                            wz * ((1 - wx) * in_at(in, s + nx + n) + wx * in_at(in, s + 1 + nx + n)));
+        OutPack<OutT, V> o;
+        for (int v = 0; v < V; v++) {
+            float val = 0.0f;
+            if (valid[v]) {
+                const uint64_t q = s[v];
+                if (!IS3D) {
+                    val = (1 - wy[v]) * ((1 - wx[v]) * in_at(in, q) + wx[v] * in_at(in, q + 1)) +
+                          wy[v] * ((1 - wx[v]) * in_at(in, q + n) + wx[v] * in_at(in, q + 1 + n));
+                } else {
+                    val = (1 - wy[v]) * ((1 - wz[v]) * ((1 - wx[v]) * in_at(in, q) + wx[v] * in_at(in, q + 1)) +
+                                         wz[v] * ((1 - wx[v]) * in_at(in, q + nx) + wx[v] * in_at(in, q + 1 + nx))) +
+                          wy[v] * ((1 - wz[v]) * ((1 - wx[v]) * in_at(in, q + n) + wx[v] * in_at(in, q + 1 + n)) +
+                                   wz[v] * ((1 - wx[v]) * in_at(in, q + nx + n) +
+                                            wx[v] * in_at(in, q + 1 + nx + n)));
+                }
             }
+            o.v[v] = clamp_out<OutT>(val);
         }
-        out[i] = clamp_out<OutT>(val);
+        *reinterpret_cast<OutPack<OutT, V>*>(out + i0) = o;
+    }
+}
+
+// 2D with 4 pixels per thread (the packed case), input staged in LDS.  A
+// block's 1024 consecutive output pixels read a narrow range of input
+// elements (a few sample rows of a linear scan); the block finds that range
+// once and copies it per frame, converted to float as the direct kernel
+// converts each sample, into one of two LDS windows (one barrier per frame;
+// the next frame's window is loaded into registers meanwhile).  The window
+// holds floats so that neighbouring samples are read at 4-byte aligned LDS
+// addresses.  The four neighbours of each pixel are the elements the direct
+// kernel reads, so the results are identical.  A block whose range exceeds
+// the window (wide sector rows) reads the input directly.
+constexpr uint32_t kScWin = 1024;            // elements per window; two windows per block
+constexpr int kScPer = kScWin / 256;         // window elements per thread
+
+template <typename InT, typename OutT>
+__global__ __launch_bounds__(256) void scan_convert_lds_kernel(const ConvArgs a) {
+    constexpr int V = 4;
+    __shared__ float win[2][kScWin];
+    __shared__ uint64_t red_lo[4], red_hi[4];
+    const uint64_t i0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * V;
+    const bool live = i0 < a.n_pixels;   // n_pixels is a multiple of V
+    const uint64_t n = a.n_rx;
+    uint64_t s[V];
+    bool valid[V];
+    float wx[V], wy[V];
+    uint64_t lo = ~0ull, hi = 0;
+    for (int v = 0; v < V; v++) {
+        const uint64_t i = i0 + v;
+        const uint32_t s32 = live ? a.idx[i] : kOutside;
+        s[v] = s32;
+        // the last sample row has no successor: the reference's 3D guard, used in 2D too
+        valid[v] = s32 != kOutside && s[v] + 1 + n < a.in_frame;
+        wx[v] = valid[v] ? a.wx[i] : 0.0f;
+        wy[v] = valid[v] ? a.wy[i] : 0.0f;
+        if (valid[v]) {
+            lo = min(lo, s[v]);
+            hi = max(hi, s[v] + 1 + n);
+        }
+    }
+    // block range [lo, hi] of the elements read
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, static_cast<uint64_t>(__shfl_xor(static_cast<long long>(lo), o, 64)));
+        hi = max(hi, static_cast<uint64_t>(__shfl_xor(static_cast<long long>(hi), o, 64)));
+    }
+    const uint32_t wave = threadIdx.x / 64;
+    if (threadIdx.x % 64 == 0) {
+        red_lo[wave] = lo;
+        red_hi[wave] = hi;
+    }
+    __syncthreads();
+    lo = min(min(red_lo[0], red_lo[1]), min(red_lo[2], red_lo[3]));
+    hi = max(max(red_hi[0], red_hi[1]), max(red_hi[2], red_hi[3]));
+    const bool staged = lo <= hi && hi - lo < kScWin;
+    const uint32_t extent = staged ? static_cast<uint32_t>(hi - lo + 1) : 0u;
+
+    const InT* in = static_cast<const InT*>(a.in);
+    OutT* out = static_cast<OutT*>(a.out);
+    float pre[kScPer];
+    auto fetch = [&](const InT* src) {
+        for (int j = 0; j < kScPer; j++) {
+            const uint32_t k = threadIdx.x + static_cast<uint32_t>(j) * 256u;
+            pre[j] = k < extent ? static_cast<float>(src[lo + k]) : 0.0f;
+        }
+    };
+    if (staged) fetch(in);
+    for (uint32_t f = 0; f < a.num_frames; f++, in += a.in_frame, out += a.n_pixels) {
+        const uint32_t b = f & 1u;
+        if (staged) {
+            for (int j = 0; j < kScPer; j++) {
+                const uint32_t k = threadIdx.x + static_cast<uint32_t>(j) * 256u;
+                if (k < extent) win[b][k] = pre[j];
+            }
+            __syncthreads();
+            if (f + 1 < a.num_frames) fetch(in + a.in_frame);
+        }
+        OutPack<OutT, V> o;
+        for (int v = 0; v < V; v++) {
+            float val = 0.0f;
+            if (valid[v]) {
+                if (staged) {
+                    const uint32_t q = static_cast<uint32_t>(s[v] - lo);
+                    val = (1 - wy[v]) * ((1 - wx[v]) * win[b][q] + wx[v] * win[b][q + 1]) +
+                          wy[v] * ((1 - wx[v]) * win[b][q + n] + wx[v] * win[b][q + 1 + n]);
+                } else {
+                    const uint64_t q = s[v];
+                    val = (1 - wy[v]) * ((1 - wx[v]) * in_at(in, q) + wx[v] * in_at(in, q + 1)) +
+                          wy[v] * ((1 - wx[v]) * in_at(in, q + n) + wx[v] * in_at(in, q + 1 + n));
+                }
+            }
+            o.v[v] = clamp_out<OutT>(val);
+        }
+        if (live) *reinterpret_cast<OutPack<OutT, V>*>(out + i0) = o;
     }
 }
 
@@ -592,12 +703,20 @@ extern "C" int supra_scanconv_run(const supra_scanconv* sc, const void* scanline
     a.n_rx = sc->layout[0] * sc->layout[1];
     a.in_frame = static_cast<uint64_t>(sc->n_samples) * a.n_rx;
     a.num_frames = num_frames;
-    const dim3 grid(static_cast<uint32_t>((a.n_pixels + 255) / 256)), block(256);
+    const size_t out_size = out_dtype == SUPRA_FLOAT32 ? 4 : (out_dtype == SUPRA_INT16 ? 2 : 1);
+    const bool packed = a.n_pixels % 4 == 0 && reinterpret_cast<uintptr_t>(out) % (4 * out_size) == 0;
+    const uint64_t threads = packed ? a.n_pixels / 4 : a.n_pixels;
+    const dim3 grid(static_cast<uint32_t>((threads + 255) / 256)), block(256);
     hipStream_t s = static_cast<hipStream_t>(stream);
 #define SUPRA_SC_LAUNCH(IN, OUT)                                                                                       \
     do {                                                                                                              \
-        if (sc->is3d) hipLaunchKernelGGL((scan_convert_kernel<IN, OUT, true>), grid, block, 0, s, a);                   \
-        else hipLaunchKernelGGL((scan_convert_kernel<IN, OUT, false>), grid, block, 0, s, a);                           \
+        if (packed) {                                                                                                 \
+            if (sc->is3d) hipLaunchKernelGGL((scan_convert_kernel<IN, OUT, true, 4>), grid, block, 0, s, a);          \
+            else hipLaunchKernelGGL((scan_convert_lds_kernel<IN, OUT>), grid, block, 0, s, a);                        \
+        } else {                                                                                                      \
+            if (sc->is3d) hipLaunchKernelGGL((scan_convert_kernel<IN, OUT, true, 1>), grid, block, 0, s, a);          \
+            else hipLaunchKernelGGL((scan_convert_kernel<IN, OUT, false, 1>), grid, block, 0, s, a);                  \
+        }                                                                                                             \
     } while (0)
 #define SUPRA_SC_OUT(IN)                                                                                              \
     do {                                                                                                              \
